@@ -1,0 +1,193 @@
+"""Throughput of the visibility + reconstruction path on one MI355X.
+
+Workload (one "step"): the plain FP64 form over the whole segment schedule of
+HEALPix NESTED Nside=128 (196,608 pixels, 768 static 256-pixel tiles),
+16 antennas on seeded circular Earth orbits (120 baselines), 512 epochs 10 s
+apart, 30 MHz, 16 equal segments, tile-cone culling, blockage=1, occ_mode=2,
+uv-mirror reconstruction into the HEALPix map and a 512x512 lm image.  A step
+is: clear accumulators, for every segment generate geometry, Viss, phase
+correction + conjugate fill, grouping and reconstruction, then normalise on
+the device.  Inputs (sky, resident directions) are in HBM before timing.
+
+Metric: pixel-baseline Gterms/s = (sum_seg npix*segN_half +
+sum_seg (npix + n_img)*nkeys_seg) / t / 1e9, counting culled terms too.
+The mixed form (FP64 phase, FP32 hardware sincos, FP32 block sums folded in
+FP64) is timed on the same workload and its error is stated against the
+plain FP64 result (which matches the CPU FP64 oracle to 1e-9).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pixel-baseline Gterms/s Viss+recon, plain FP64 vs mixed at stated FP64 err"
+WORKLOAD = dict(nside=128, n_ant=16, n_epochs=512, freq_hz=3e7, segs=16, img_n=512, tile_pix=256)
+WORKLOAD_NAME = ("HEALPix NESTED Nside=128 (196,608 px, 768 tiles), 16 antennas / 120 baselines on seeded "
+                 "circular orbits r in [6900,7100] km, 512 epochs x 10 s, 30 MHz, 16 segments, "
+                 "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 512x512 lm image")
+FP64_PEAK_TFLOPS = 78.6
+# FP64 flops of one evaluated Viss term in the plain form (shared-header
+# formulas): phase 1 mul + 2 fma = 5; turn reduction 3; 2*pi scaling
+# fma + mul = 3; y^2 1; sin poly 8 fma + mul + fma = 19; cos poly 9 fma = 18;
+# accumulation 2 fma = 4.
+VISS_TERM_FLOPS_PLAIN = 53
+
+
+def _cfg(precision):
+    from viss_optimize_ws_amd import default_config
+
+    return default_config(1, precision=precision, **WORKLOAD)
+
+
+def _plan_counts(pipe):
+    """One instrumented pass over the schedule: per-segment nkeys, evaluated
+    (non-culled) Viss terms, and the final maps for error statements."""
+    per_seg = []
+    TP = pipe.cfg.tile_pix
+    npix = pipe.plan.npix
+    tile_sizes = np.full(pipe.plan.ntile, TP, np.int64)
+    tile_sizes[-1] = npix - TP * (pipe.plan.ntile - 1)
+    pipe.reset_accumulators()
+    for t0, tl in pipe.schedule():
+        pipe.gen_segment(t0, tl)
+        cls = pipe.tile_classes()
+        evaluated = int(((cls != 2).astype(np.int64) * tile_sizes[None, :]).sum())
+        vh = pipe.viss_half()
+        pipe.conj_fill()
+        info = pipe.recon_segment()
+        per_seg.append(dict(n_half=info["n_half"], nkeys=info["nkeys"], evaluated=evaluated, vraw=vh))
+    maps = pipe.finalize()
+    return per_seg, maps
+
+
+def _time_steps(pipe, steps, warmup):
+    lib = pipe._lib
+    for _ in range(warmup):
+        pipe.reset_accumulators()
+        pipe.run_all()
+        lib.vw_finalize(pipe._ctx, None, None, None, None)
+    pipe.timings()  # clear
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.reset_accumulators()
+        pipe.run_all()
+        rc = lib.vw_finalize(pipe._ctx, None, None, None, None)
+        assert rc == 0
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, pipe.timings()
+
+
+def _cpu_baseline(sky, threads):
+    """CPU FP64 oracle on a bounded sample: the first segment in full (Viss of
+    all its half rows, reconstruction of every HEALPix pixel and image
+    point)."""
+    from oracle.oracle import Oracle
+
+    cfg = _cfg(0)
+    with Oracle(cfg) as o:
+        o.set_sky(sky)
+        t0, tl = o.schedule()[0]
+        o.gen_segment(t0, tl)
+        s = time.perf_counter()
+        o.viss_half()
+        o.conj_fill()
+        info = o.recon_segment()
+        dt = time.perf_counter() - s
+        terms = o.plan.npix * info["n_half"] + (o.plan.npix + o.plan.n_img_pts) * info["nkeys"]
+    return dict(value=terms / dt / 1e9, unit="Gterms/s", cores=threads, kind="port",
+                sample=f"segment 1 of 16 in full: {terms:.3e} terms in {dt:.2f} s (oracle, OpenMP FP64)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from viss_optimize_ws_amd import Pipeline, make_sky
+
+    sky_seed = 20240
+    with Pipeline(_cfg(0)) as plain:
+        npix, nimg = plain.plan.npix, plain.plan.n_img_pts
+        sky = make_sky(sky_seed, npix, 0)
+        plain.set_sky(sky)
+        per_seg, maps_plain = _plan_counts(plain)
+        t_viss = sum(npix * s["n_half"] for s in per_seg)
+        t_recon = sum((npix + nimg) * s["nkeys"] for s in per_seg)
+        terms = t_viss + t_recon
+        evaluated = sum(s["evaluated"] for s in per_seg)
+        sec, tim = _time_steps(plain, args.steps, args.warmup)
+
+    value = terms / sec / 1e9
+    nl = max(tim["n_viss_launch"], 1)
+    viss_avg_ms = tim["viss_kernel_ms"] / nl
+    eval_per_launch = evaluated / len(per_seg)
+    achieved = VISS_TERM_FLOPS_PLAIN * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gterms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic: log-normal(0,1) sky seed {sky_seed}, circular orbits seed 42",
+        "config": {"workload": WORKLOAD_NAME, **WORKLOAD, "form": "plain FP64",
+                   "terms_per_step": terms, "viss_terms": t_viss, "recon_terms": t_recon},
+        "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in tim.items() if k.endswith("_ms")},
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "FP64 vector ALU (VALU-bound direct sum; published FP64 peak)",
+            "kernel": "k_viss (plain)",
+            "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_term": VISS_TERM_FLOPS_PLAIN,
+            "evaluated_terms_per_launch": int(eval_per_launch),
+            "avg_launch_ms": round(viss_avg_ms, 4),
+        },
+    }
+    if not args.no_mixed:
+        with Pipeline(_cfg(1)) as mixed:
+            mixed.set_sky(sky)
+            per_seg_m, maps_m = _plan_counts(mixed)
+            sec_m, tim_m = _time_steps(mixed, args.steps, args.warmup)
+        from_err = {}
+        vr_p = np.concatenate([s["vraw"] for s in per_seg])
+        vr_m = np.concatenate([s["vraw"] for s in per_seg_m])
+        for name, a, b in (("viss_half", vr_m, vr_p), ("C_healpix", maps_m[0], maps_plain[0]),
+                           ("lm_image", maps_m[2], maps_plain[2])):
+            den = np.max(np.abs(b))
+            from_err[name] = {"max_rel": float(np.max(np.abs(a - b)) / den),
+                              "rms_rel": float(np.sqrt(np.mean(np.abs(a - b) ** 2)) / np.sqrt(np.mean(np.abs(b) ** 2)))}
+        out["mixed"] = {"value": round(terms / sec_m / 1e9, 4), "unit": "Gterms/s", "ms_per_step": round(sec_m * 1e3, 4),
+                        "speedup_vs_plain": round(sec / sec_m, 3), "dtype": "f64 phase + f32 sincos/sum",
+                        "err_vs_fp64": from_err,
+                        "viss_kernel_avg_ms": round(tim_m["viss_kernel_ms"] / max(tim_m["n_viss_launch"], 1), 4)}
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = _cpu_baseline(sky, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+"""gfx950 path vs the CPU FP64 oracle through the C-ABI (needs a GPU)."""
+import numpy as np
+import pytest
+
+from _lockstep import MIXED_TOL, PLAIN_TOL, config, rel_err, rms_rel_err, run_both
+from oracle.oracle import Oracle
+from viss_optimize_ws_amd import Pipeline
+
+pytestmark = pytest.mark.gpu
+
+
+def _sky(npix, seed=3):
+    return np.random.default_rng(seed).lognormal(0.0, 1.0, npix)
+
+
+# configs[0] of the workload set: Nside 32, 8 antennas, 64 epochs, 10 MHz,
+# 4 segments, blockage=1, occ_mode=2, 256^2 image
+C1 = dict(nside=32, n_ant=8, n_epochs=64, freq_hz=1e7, segs=4, img_n=256)
+
+
+@pytest.mark.parametrize("nside,tp", [(4, 256), (32, 256), (32, 512), (64, 256)])
+def test_resident_geometry_bitexact(nside, tp):
+    cfg = config(nside=nside, tile_pix=tp, n_epochs=8, segs=1, img_n=48)
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        for a, b in zip(g.lmn(), o.lmn()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(g.tiles(), o.tiles())
+        assert np.array_equal(g.tiles(img=True), o.tiles(img=True))
+        for a, b in zip(g.img_points(), o.img_points()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(g.positions(), o.positions())
+        assert g.plan.bl_max_m == o.plan.bl_max_m
+
+
+@pytest.mark.parametrize("geom", ["circular", "lunar"])
+def test_segment_geometry_bitexact(geom):
+    if geom == "lunar":
+        cfg = config(geometry=0, nside=8, freq_hz=1e6, segs=10, img_n=0)
+    else:
+        cfg = config(**C1)
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        assert np.array_equal(g.positions(), o.positions())
+        for t0, tl in g.schedule()[:2]:
+            gg = g.gen_segment(t0, tl, geometry=True)
+            go = o.gen_segment(t0, tl, geometry=True)
+            for k in gg:
+                assert np.array_equal(gg[k], go[k]), k
+            for a, b in zip(g.rows(), o.rows()):
+                assert np.array_equal(a, b)
+            assert np.array_equal(g.tile_classes(), o.tile_classes())
+            if cfg.img_n:
+                assert np.array_equal(g.tile_classes(img=True), o.tile_classes(img=True))
+            nr = min(g.nh, 64)
+            assert np.array_equal(g.masks(0, nr), o.masks(0, nr))
+
+
+def test_viss_counts_and_masks_c1():
+    cfg = config(**C1)
+    sky = _sky(12 * 32 * 32)
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        g.set_sky(sky)
+        o.set_sky(sky)
+        t0, tl = g.schedule()[1]
+        g.gen_segment(t0, tl)
+        o.gen_segment(t0, tl)
+        vo, co = o.viss_half(counts=True)
+        cg = g.viss_counts()
+        assert np.array_equal(cg, co)
+        # the effective mask equals the pure per-pixel predicate (cone invariant)
+        assert np.array_equal(o.masks(0, o.nh), o.predicate(0, o.nh))
+        vg = g.viss_half()
+        assert rel_err(vg, vo) <= PLAIN_TOL
+
+
+def _seg_checks(tol, exact_groups=True):
+    def chk(g, o, k, d):
+        assert rel_err(d["vg"], d["vo"]) <= tol, ("viss", k, rel_err(d["vg"], d["vo"]))
+        assert rel_err(d["hg"], d["ho"]) <= tol
+        assert rel_err(d["fg"], d["fo"]) <= tol
+        ig, io = d["ig"], d["io"]
+        for key in ("fa", "fb", "denom", "du", "res", "half", "nuniq", "nkeys"):
+            assert ig[key] == io[key], (key, ig[key], io[key])
+        if exact_groups:
+            _, kg, og, sg, rg = g.groups()
+            _, ko, oo, so, ro = o.groups()
+            assert np.array_equal(kg, ko) and np.array_equal(og, oo)
+            assert np.array_equal(sg, so) and np.array_equal(rg, ro)
+
+    return chk
+
+
+@pytest.mark.parametrize(
+    "variant",
+    [
+        dict(),
+        dict(tile_pix=512),
+        dict(recon_mode=1),
+        dict(occ_mode=0),
+        dict(occ_mode=1),
+        dict(occ_mode=3),
+        dict(recon_mode=1, occ_mode=3),
+        dict(blockage=0),
+        dict(recon_mode=1, blockage=0),
+    ],
+    ids=lambda v: ",".join(f"{k}={x}" for k, x in v.items()) or "default",
+)
+def test_pipeline_plain_c1(variant):
+    kw = dict(C1)
+    kw.update(variant)
+    cfg = config(**kw)
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 32 * 32), _seg_checks(PLAIN_TOL))
+    assert np.array_equal(Wg, Wo)
+    assert np.array_equal(WIg, WIo)
+    assert rel_err(Cg, Co) <= PLAIN_TOL
+    assert rel_err(Ig, Io) <= PLAIN_TOL
+
+
+def test_pipeline_lunar_reference_geometry():
+    # reference formation (1 MHz band), small sky, 3 of its segments
+    cfg = config(geometry=0, nside=8, freq_hz=1e6, segs=20, img_n=32)
+    sky = _sky(12 * 8 * 8, seed=5)
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, sky, _seg_checks(PLAIN_TOL), segments={0, 7, 19})
+    assert np.array_equal(Wg, Wo)
+    assert rel_err(Cg, Co) <= PLAIN_TOL
+    assert rel_err(Ig, Io) <= PLAIN_TOL
+
+
+def test_pipeline_mixed_c1():
+    cfg = config(precision=1, **C1)
+    errs = {}
+
+    def chk(g, o, k, d):
+        errs.setdefault("viss", []).append(rel_err(d["vg"], d["vo"]))
+        assert d["ig"]["nuniq"] == d["io"]["nuniq"]
+
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 32 * 32), chk)
+    assert max(errs["viss"]) <= MIXED_TOL, errs
+    assert np.array_equal(Wg, Wo)
+    assert rel_err(Cg, Co) <= MIXED_TOL, rel_err(Cg, Co)
+    assert rel_err(Ig, Io) <= MIXED_TOL, rel_err(Ig, Io)
+    print("mixed C1 errors: viss max", max(errs["viss"]), "C", rel_err(Cg, Co), rms_rel_err(Cg, Co),
+          "img", rel_err(Ig, Io), rms_rel_err(Ig, Io))

@@ -1,0 +1,966 @@
+/*
+ * vissws.hip -- host orchestration and C-ABI of the gfx950 path
+ * (include/vissws.h).  One HIP stream per context at default priority; all
+ * device memory is allocated once in vw_create and sized for the longest
+ * segment, so the segment loop performs no allocation.  Stage functions
+ * enqueue work; the public entry points synchronise before returning.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/vissws.h"
+#include "vw_kernels.h"
+#include "vw_shared.h"
+
+#define VW_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_create_err;
+
+struct EvPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+enum Stage { ST_GEN = 0, ST_VISS, ST_CONJ, ST_GROUP, ST_RECON, ST_FIN, ST_VISSK, ST_RECONK, ST_N };
+
+struct Ctx {
+  vw_config cfg{};
+  std::string err;
+  /* plan */
+  int64_t nside = 0, npix = 0, TP = 256, ntile = 0;
+  int A = 0, P = 0, T = 0, segs = 0, max_tlen = 0;
+  std::vector<int32_t> seg_t0, seg_tlen;
+  double lambda = 0, bl_max = 0, cosphi_glob = NAN, pix_area = 0;
+  bool per_ant = false;
+  int64_t orbit_res = 0, seg_len = 0, tw = 0, day_offset = 0, pc = 0;
+  std::vector<int64_t> t1s;
+  std::vector<double> circ; /* [7][A] */
+  std::vector<int32_t> pa, pb;
+  int64_t ngp = 0, ngt = 0;
+  std::vector<int64_t> gidx;
+  int64_t max_nh = 0, nh_pad = 0, nchunk_max = 1;
+  /* device */
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  double *d_l = nullptr, *d_m = nullptr, *d_n = nullptr, *d_nm1 = nullptr, *d_cone = nullptr;
+  double* d_B = nullptr;
+  float* d_Bf = nullptr;
+  int64_t* d_gidx = nullptr;
+  double *d_gl = nullptr, *d_gm = nullptr, *d_gn = nullptr, *d_gcone = nullptr;
+  double* d_pos = nullptr;
+  int64_t* d_t1s = nullptr;
+  double* d_circ = nullptr;
+  int32_t *d_pa = nullptr, *d_pb = nullptr;
+  RowRec* d_rows = nullptr;
+  double* d_ant8 = nullptr;
+  uint8_t *d_cls = nullptr, *d_gcls = nullptr;
+  double2* d_part = nullptr;
+  int64_t* d_cpart = nullptr;
+  int64_t* d_counts = nullptr;
+  double2 *d_vraw = nullptr, *d_vh = nullptr, *d_vfull = nullptr;
+  int64_t *d_keys = nullptr, *d_keys_sorted = nullptr, *d_gkeys = nullptr;
+  int32_t *d_idx_sorted = nullptr, *d_goff = nullptr;
+  GroupRec* d_grec = nullptr;
+  DevInfo* d_info = nullptr;
+  double *d_Cacc = nullptr, *d_Gacc = nullptr;
+  uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
+  double *d_out = nullptr, *d_iout = nullptr;
+  uint32_t *d_wout = nullptr, *d_iwout = nullptr;
+  double* d_full = nullptr;
+  /* segment state */
+  int t0 = -1, tlen = 0;
+  int64_t nh = 0;
+  bool have_sky = false, have_seg = false, have_viss = false, have_conj = false, have_groups = false;
+  /* timing */
+  std::vector<EvPair> ev[ST_N];
+  size_t ev_used[ST_N] = {};
+  vw_timings tim{};
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+#define HC(x)                                                                              \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define KC() HC(hipGetLastError())
+
+template <typename T>
+int dalloc(Ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  HC(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return VW_OK;
+}
+
+int round_away_f(float x) { return (x >= 0.0f) ? (int)floorf(x + 0.5f) : (int)ceilf(x - 0.5f); }
+
+inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+/* event bracket for one stage launch group */
+hipEvent_t ev_begin(Ctx* c, Stage s) {
+  auto& v = c->ev[s];
+  if (c->ev_used[s] == v.size()) {
+    EvPair p;
+    (void)hipEventCreate(&p.a);
+    (void)hipEventCreate(&p.b);
+    v.push_back(p);
+  }
+  EvPair& p = v[c->ev_used[s]];
+  (void)hipEventRecord(p.a, c->st);
+  return p.b;
+}
+void ev_end(Ctx* c, Stage s) {
+  (void)hipEventRecord(c->ev[s][c->ev_used[s]].b, c->st);
+  c->ev_used[s]++;
+}
+
+/* after a synchronise: fold recorded brackets into the timings */
+void collect_timings(Ctx* c) {
+  double* dst[ST_N] = {&c->tim.gen_ms, &c->tim.viss_ms, &c->tim.conj_ms, &c->tim.group_ms,
+                       &c->tim.recon_ms, &c->tim.finalize_ms, &c->tim.viss_kernel_ms, &c->tim.recon_kernel_ms};
+  for (int s = 0; s < ST_N; ++s) {
+    for (size_t k = 0; k < c->ev_used[s]; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, c->ev[s][k].a, c->ev[s][k].b);
+      *dst[s] += ms;
+    }
+    if (s == ST_VISSK) c->tim.n_viss_launch += (int32_t)c->ev_used[s];
+    if (s == ST_RECONK) c->tim.n_recon_launch += (int32_t)c->ev_used[s];
+    c->ev_used[s] = 0;
+  }
+}
+
+int sync(Ctx* c) {
+  HC(hipStreamSynchronize(c->st));
+  collect_timings(c);
+  return VW_OK;
+}
+
+/* ---------------- plan ---------------- */
+int make_plan(Ctx* c, const vw_config* cfg) {
+  c->cfg = *cfg;
+  c->cfg.seg_tlen = nullptr;
+  if (cfg->nside <= 0 || cfg->nside > (1 << 20) || !(cfg->freq_hz > 0)) return c->fail(VW_ERR_ARG, "bad nside/freq");
+  if (cfg->tile_pix != 256 && cfg->tile_pix != 512) return c->fail(VW_ERR_ARG, "tile_pix must be 256 or 512");
+  if (cfg->geometry != VW_GEOM_LUNAR8 && cfg->geometry != VW_GEOM_CIRCULAR) return c->fail(VW_ERR_ARG, "geometry");
+  if (cfg->geometry == VW_GEOM_LUNAR8 && cfg->n_ant != 8) return c->fail(VW_ERR_ARG, "lunar geometry has 8 antennas");
+  if (cfg->n_ant < 2 || cfg->n_ant > 256) return c->fail(VW_ERR_ARG, "n_ant out of range");
+  if (cfg->precision != VW_PREC_PLAIN && cfg->precision != VW_PREC_MIXED) return c->fail(VW_ERR_ARG, "precision");
+  if (cfg->recon_mode != VW_RECON_MIRROR && cfg->recon_mode != VW_RECON_FULL) return c->fail(VW_ERR_ARG, "recon_mode");
+  if (cfg->blockage != 0 && cfg->blockage != 1) return c->fail(VW_ERR_ARG, "blockage must be 0 or 1");
+  if (cfg->img_n < 0 || cfg->img_n > 16384) return c->fail(VW_ERR_ARG, "img_n out of range");
+  c->nside = cfg->nside;
+  c->npix = 12 * c->nside * c->nside;
+  c->TP = cfg->tile_pix;
+  c->ntile = (c->npix + c->TP - 1) / c->TP;
+  c->A = cfg->n_ant;
+  c->P = c->A * (c->A - 1) / 2;
+  c->lambda = 3e8 / cfg->freq_hz;
+  c->pix_area = (4.0 * VW_PI) / (double)c->npix;
+  c->per_ant = cfg->occ_altitude_km < 0.0;
+  c->cosphi_glob = c->per_ant ? NAN : vw_cosphi_from_altitude(cfg->occ_radius_km, cfg->occ_altitude_km);
+  if (cfg->geometry == VW_GEOM_LUNAR8) {
+    /* orbit_online.cuh:238-241 */
+    c->orbit_res = (int64_t)std::ceil((2.0 * VW_PI) * (100e3 / c->lambda));
+    c->pc = round_away_f(24.0f / 2.3f);
+    c->seg_len = c->orbit_res / 3;
+    c->T = (int)(c->pc * c->seg_len);
+  } else {
+    if (cfg->n_epochs <= 0) return c->fail(VW_ERR_ARG, "n_epochs must be > 0");
+    if (!(cfg->r_max_km >= cfg->r_min_km) || !(cfg->r_min_km > 0) || !(cfg->gm_km3_s2 > 0))
+      return c->fail(VW_ERR_ARG, "circular orbit parameters");
+    c->T = cfg->n_epochs;
+    /* per-antenna draws: radius, inclination, RAAN, phase (u01 as orbit_online.cuh:324) */
+    std::mt19937_64 rng(cfg->orbit_seed);
+    auto u01 = [&]() { return (double)(rng() >> 11) * (1.0 / 9007199254740992.0); };
+    int A = c->A;
+    c->circ.assign(7 * (size_t)A, 0.0);
+    for (int a = 0; a < A; ++a) {
+      double r = cfg->r_min_km + (cfg->r_max_km - cfg->r_min_km) * u01();
+      double inc = 180.0 * u01();
+      double raan = 360.0 * u01();
+      double ph = 360.0 * u01();
+      c->circ[a] = r;
+      c->circ[A + a] = ph / 360.0;
+      c->circ[2 * A + a] = std::sqrt(cfg->gm_km3_s2 / ((r * r) * r)) * VW_INV_TWOPI;
+      vw_sincos_turn(raan / 360.0, &c->circ[3 * A + a], &c->circ[4 * A + a]);
+      vw_sincos_turn(inc / 360.0, &c->circ[5 * A + a], &c->circ[6 * A + a]);
+    }
+  }
+  /* schedule: explicit or equal split (main_recon_online.cu:123-139) */
+  c->seg_t0.clear();
+  c->seg_tlen.clear();
+  if (cfg->n_seg_explicit > 0) {
+    if (!cfg->seg_tlen) return c->fail(VW_ERR_ARG, "seg_tlen is NULL");
+    int64_t cur = 0;
+    for (int k = 0; k < cfg->n_seg_explicit; ++k) {
+      if (cfg->seg_tlen[k] <= 0) return c->fail(VW_ERR_ARG, "segment length <= 0");
+      c->seg_t0.push_back((int32_t)cur);
+      c->seg_tlen.push_back(cfg->seg_tlen[k]);
+      cur += cfg->seg_tlen[k];
+    }
+    if (cur > c->T) return c->fail(VW_ERR_ARG, "segments exceed the epochs of the run");
+  } else {
+    int segs = cfg->segs <= 0 ? 1 : cfg->segs;
+    if (segs > c->T) segs = c->T;
+    int base = c->T / segs, rem = c->T % segs, cur = 0;
+    for (int k = 0; k < segs; ++k) {
+      int len = base + (k < rem ? 1 : 0);
+      c->seg_t0.push_back(cur);
+      c->seg_tlen.push_back(len);
+      cur += len;
+    }
+  }
+  c->segs = (int)c->seg_t0.size();
+  c->max_tlen = *std::max_element(c->seg_tlen.begin(), c->seg_tlen.end());
+  c->pa.resize(c->P);
+  c->pb.resize(c->P);
+  for (int mm = 0, k = 0; mm < c->A - 1; ++mm)
+    for (int nn = mm + 1; nn < c->A; ++nn, ++k) { c->pa[k] = nn; c->pb[k] = mm; }
+  /* lm grid point order: 16x16 blocks, row-major blocks */
+  c->gidx.clear();
+  if (cfg->img_n > 0) {
+    int64_t N = cfg->img_n;
+    for (int64_t bi = 0; bi < N; bi += 16)
+      for (int64_t bj = 0; bj < N; bj += 16)
+        for (int64_t ii = bi; ii < bi + 16 && ii < N; ++ii)
+          for (int64_t jj = bj; jj < bj + 16 && jj < N; ++jj) {
+            int64_t a = 2 * jj + 1 - N, b = 2 * ii + 1 - N;
+            if (a * a + b * b < N * N) c->gidx.push_back(ii * N + jj);
+          }
+  }
+  c->ngp = (int64_t)c->gidx.size();
+  c->ngt = (c->ngp + c->TP - 1) / c->TP;
+  c->max_nh = (int64_t)c->max_tlen * c->P;
+  c->nh_pad = (c->max_nh + 63) / 64 * 64;
+  if (c->max_nh > (int64_t)INT32_MAX / 4) return c->fail(VW_ERR_RANGE, "segment too long for 32-bit row indices");
+  return VW_OK;
+}
+
+/* Viss launch shape for nh half rows */
+struct VissShape {
+  int rpw;
+  int64_t tpc, nchunk, nrb, bpc, nblocks;
+};
+VissShape viss_shape(const Ctx* c, int64_t nh) {
+  VissShape s;
+  bool mixed = c->cfg.precision == VW_PREC_MIXED;
+  s.rpw = mixed ? 32 : 16;
+  s.nrb = (nh + s.rpw - 1) / s.rpw;
+  const int64_t target_waves = 8192;
+  int64_t nchunk = (target_waves + s.nrb - 1) / s.nrb;
+  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, c->ntile));
+  s.tpc = (c->ntile + nchunk - 1) / nchunk;
+  const int64_t tpc_cap = mixed ? 16 : 64; /* bounds the FP32 block length per lane */
+  if (s.tpc > tpc_cap) s.tpc = tpc_cap;
+  s.nchunk = (c->ntile + s.tpc - 1) / s.tpc;
+  s.bpc = (s.nrb + 3) / 4;
+  s.nblocks = s.nchunk * s.bpc;
+  return s;
+}
+
+int upload_day(Ctx* c) {
+  /* lunar t1s windows (orbit_online.cuh:317-333) */
+  if (c->cfg.geometry == VW_GEOM_LUNAR8) {
+    int64_t OR = c->orbit_res;
+    c->tw = round_away_f((float)OR * 7.0f * 24.0f / 2.3f);
+    int mod14 = ((c->cfg.day - 1) % 14 + 14) % 14;
+    c->day_offset = round_away_f((float)mod14 * (float)OR * 24.0f / 2.3f);
+    uint64_t day_seed = c->cfg.orbit_seed ^ (uint64_t)(int64_t)c->cfg.day * 0x9E3779B97F4A7C15ULL;
+    std::mt19937_64 rng(day_seed);
+    c->t1s.assign((size_t)c->pc, 0);
+    for (int64_t g = 0; g < c->pc; ++g) {
+      int64_t os = g * OR, oe = (g + 1) * OR - 1;
+      double u01 = (double)(rng() >> 11) * (1.0 / 9007199254740992.0);
+      int64_t t1 = os + round_away_f((float)(u01 * (double)OR * (2.0 / 3.0)));
+      int64_t t2 = t1 + c->seg_len - 1;
+      if (t2 > oe) { t2 = oe; t1 = t2 - c->seg_len + 1; }
+      if (t1 < os) t1 = os;
+      if (t1 + c->seg_len - 1 > oe) t1 = oe - c->seg_len + 1;
+      c->t1s[g] = t1;
+    }
+    HC(hipMemcpyAsync(c->d_t1s, c->t1s.data(), sizeof(int64_t) * c->t1s.size(), hipMemcpyHostToDevice, c->st));
+  }
+  DayArgs a{};
+  a.geometry = c->cfg.geometry;
+  a.T = c->T;
+  a.A = c->A;
+  a.orbit_res = c->orbit_res;
+  a.seg_len = c->seg_len;
+  a.tw = c->tw;
+  a.day_offset = c->day_offset;
+  a.day = c->cfg.day;
+  a.t1s = c->d_t1s;
+  a.circ = c->d_circ;
+  a.dt = c->cfg.dt_s;
+  int64_t TA = (int64_t)c->T * c->A;
+  k_day_positions<<<blocks_for(TA), 256, 0, c->st>>>(a, c->d_pos);
+  KC();
+  /* automatic bl_max: lunar 100 km (main_recon_online.cu:90); circular: the
+   * longest 3-D baseline of the run (exact max of device positions) */
+  if (c->cfg.bl_max_m > 0.0) {
+    c->bl_max = c->cfg.bl_max_m;
+  } else if (c->cfg.geometry == VW_GEOM_LUNAR8) {
+    c->bl_max = 100e3;
+  } else {
+    std::vector<double> pos(3 * (size_t)TA);
+    HC(hipMemcpyAsync(pos.data(), c->d_pos, sizeof(double) * pos.size(), hipMemcpyDeviceToHost, c->st));
+    HC(hipStreamSynchronize(c->st));
+    double mx = 0.0;
+    for (int t = 0; t < c->T; ++t)
+      for (int j = 0; j < c->P; ++j) {
+        int64_t ka = (int64_t)t * c->A + c->pa[j], kb = (int64_t)t * c->A + c->pb[j];
+        double dx = pos[ka] - pos[kb], dy = pos[TA + ka] - pos[TA + kb], dz = pos[2 * TA + ka] - pos[2 * TA + kb];
+        mx = std::fmax(mx, std::sqrt(vw_dot3(dx, dy, dz, dx, dy, dz)) * 1000.0);
+      }
+    c->bl_max = mx;
+  }
+  return VW_OK;
+}
+
+int create_device(Ctx* c) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
+  HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  int rc;
+  const size_t np = (size_t)c->npix;
+  if ((rc = dalloc(c, &c->d_l, np)) || (rc = dalloc(c, &c->d_m, np)) || (rc = dalloc(c, &c->d_n, np)) ||
+      (rc = dalloc(c, &c->d_nm1, np)) || (rc = dalloc(c, &c->d_cone, 5 * (size_t)c->ntile)) ||
+      (rc = dalloc(c, &c->d_B, np)) || (rc = dalloc(c, &c->d_Bf, np)) || (rc = dalloc(c, &c->d_Cacc, np)) ||
+      (rc = dalloc(c, &c->d_Wacc, np)))
+    return rc;
+  k_pix2lmn<<<blocks_for(c->npix), 256, 0, c->st>>>(c->nside, c->npix, c->d_l, c->d_m, c->d_n, c->d_nm1);
+  KC();
+  if (c->TP == 256) k_tile_cone<256><<<(unsigned)c->ntile, 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->ntile, c->d_cone);
+  else k_tile_cone<512><<<(unsigned)c->ntile, 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->ntile, c->d_cone);
+  KC();
+  if (c->ngp > 0) {
+    const size_t ng = (size_t)c->ngp;
+    if ((rc = dalloc(c, &c->d_gidx, ng)) || (rc = dalloc(c, &c->d_gl, ng)) || (rc = dalloc(c, &c->d_gm, ng)) ||
+        (rc = dalloc(c, &c->d_gn, ng)) || (rc = dalloc(c, &c->d_gcone, 5 * (size_t)c->ngt)) ||
+        (rc = dalloc(c, &c->d_Gacc, ng)) || (rc = dalloc(c, &c->d_GWacc, ng)))
+      return rc;
+    HC(hipMemcpyAsync(c->d_gidx, c->gidx.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice, c->st));
+    k_grid_points<<<blocks_for(c->ngp), 256, 0, c->st>>>(c->cfg.img_n, c->ngp, c->d_gidx, c->d_gl, c->d_gm, c->d_gn);
+    KC();
+    if (c->TP == 256) k_tile_cone<256><<<(unsigned)c->ngt, 256, 0, c->st>>>(c->d_gl, c->d_gm, c->d_gn, c->ngp, c->ngt, c->d_gcone);
+    else k_tile_cone<512><<<(unsigned)c->ngt, 256, 0, c->st>>>(c->d_gl, c->d_gm, c->d_gn, c->ngp, c->ngt, c->d_gcone);
+    KC();
+  }
+  int64_t TA = (int64_t)c->T * c->A;
+  if ((rc = dalloc(c, &c->d_pos, 3 * (size_t)TA)) || (rc = dalloc(c, &c->d_t1s, (size_t)std::max<int64_t>(c->pc, 1))) ||
+      (rc = dalloc(c, &c->d_circ, 7 * (size_t)c->A)) || (rc = dalloc(c, &c->d_pa, (size_t)c->P)) ||
+      (rc = dalloc(c, &c->d_pb, (size_t)c->P)))
+    return rc;
+  HC(hipMemcpyAsync(c->d_pa, c->pa.data(), sizeof(int32_t) * c->P, hipMemcpyHostToDevice, c->st));
+  HC(hipMemcpyAsync(c->d_pb, c->pb.data(), sizeof(int32_t) * c->P, hipMemcpyHostToDevice, c->st));
+  if (!c->circ.empty())
+    HC(hipMemcpyAsync(c->d_circ, c->circ.data(), sizeof(double) * c->circ.size(), hipMemcpyHostToDevice, c->st));
+  /* segment buffers sized for the longest segment */
+  const size_t mh = (size_t)c->max_nh;
+  int64_t nch = 1;
+  for (int k = 0; k < c->segs; ++k) nch = std::max(nch, viss_shape(c, (int64_t)c->seg_tlen[k] * c->P).nchunk);
+  c->nchunk_max = nch;
+  const double cls_bytes = (double)(c->ntile + c->ngt) * (double)c->nh_pad;
+  if (cls_bytes > 32e9) return c->fail(VW_ERR_RANGE, "tile-class table exceeds 32 GB; use fewer epochs per segment");
+  if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
+      (rc = dalloc(c, &c->d_cls, (size_t)c->ntile * c->nh_pad)) ||
+      (rc = dalloc(c, &c->d_gcls, (size_t)std::max<int64_t>(c->ngt, 1) * c->nh_pad)) ||
+      (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) || (rc = dalloc(c, &c->d_vraw, mh)) ||
+      (rc = dalloc(c, &c->d_vh, mh)) || (rc = dalloc(c, &c->d_vfull, 2 * mh)) ||
+      (rc = dalloc(c, &c->d_keys, 2 * mh)) || (rc = dalloc(c, &c->d_keys_sorted, 2 * mh)) ||
+      (rc = dalloc(c, &c->d_gkeys, 2 * mh + 1)) || (rc = dalloc(c, &c->d_idx_sorted, 2 * mh)) ||
+      (rc = dalloc(c, &c->d_goff, 2 * mh + 2)) || (rc = dalloc(c, &c->d_grec, 2 * mh + 1)) ||
+      (rc = dalloc(c, &c->d_info, 1)) ||
+      (rc = dalloc(c, &c->d_out, (size_t)c->npix)) || (rc = dalloc(c, &c->d_wout, (size_t)c->npix)) ||
+      (rc = dalloc(c, &c->d_iout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))) ||
+      (rc = dalloc(c, &c->d_iwout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))))
+    return rc;
+  HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
+  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
+  HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * np, c->st));
+  if (c->ngp) {
+    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
+  }
+  if ((rc = upload_day(c))) return rc;
+  HC(hipStreamSynchronize(c->st));
+  return VW_OK;
+}
+
+/* ---------------- stages (enqueue only) ---------------- */
+int enq_gen(Ctx* c, int t0, int tlen) {
+  if (t0 < 0 || tlen <= 0 || tlen > c->max_tlen || (int64_t)t0 + tlen > c->T)
+    return c->fail(VW_ERR_ARG, "segment [t0, t0+tlen) out of range");
+  c->t0 = t0;
+  c->tlen = tlen;
+  c->nh = (int64_t)tlen * c->P;
+  ev_begin(c, ST_GEN);
+  int64_t na = (int64_t)tlen * c->A;
+  k_seg_ant<<<blocks_for(na), 256, 0, c->st>>>(c->d_pos, c->T, c->A, t0, tlen, c->per_ant ? c->cfg.occ_radius_km : 0.0,
+                                               c->cosphi_glob, c->d_ant8);
+  KC();
+  k_seg_rows<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_pos, c->T, c->A, c->P, t0, c->nh, c->d_pa, c->d_pb, c->lambda,
+                                                   c->d_rows);
+  KC();
+  k_classify<<<blocks_for(c->ntile * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, c->d_rows, c->d_ant8, c->nh,
+                                                                  c->nh_pad, c->d_cls);
+  KC();
+  if (c->ngp) {
+    k_classify<<<blocks_for(c->ngt * c->nh_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->d_rows, c->d_ant8, c->nh,
+                                                                  c->nh_pad, c->d_gcls);
+    KC();
+  }
+  ev_end(c, ST_GEN);
+  c->have_seg = true;
+  c->have_viss = c->have_conj = c->have_groups = false;
+  return VW_OK;
+}
+
+template <int TP, int RPW, bool MIXED, bool COUNT>
+void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
+  k_viss<TP, RPW, MIXED, COUNT><<<nb, 256, 0, st>>>(a);
+}
+
+int enq_viss(Ctx* c, bool count) {
+  if (!c->have_seg || !c->have_sky) return c->fail(VW_ERR_STATE, "vw_viss_half needs vw_set_sky and vw_gen_segment");
+  VissShape s = viss_shape(c, c->nh);
+  if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
+  if (count && !c->d_cpart) {
+    int rc;
+    if ((rc = dalloc(c, &c->d_cpart, (size_t)c->nchunk_max * c->nh_pad)) || (rc = dalloc(c, &c->d_counts, (size_t)c->max_nh)))
+      return rc;
+  }
+  VissArgs a{};
+  a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
+  a.npix = c->npix; a.ntile = c->ntile;
+  a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
+  a.nh = c->nh; a.nh_pad = c->nh_pad;
+  a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
+  a.part = c->d_part; a.cpart = c->d_cpart;
+  ev_begin(c, ST_VISS);
+  ev_begin(c, ST_VISSK);
+  unsigned nb = (unsigned)s.nblocks;
+  bool mixed = c->cfg.precision == VW_PREC_MIXED;
+  if (c->TP == 256) {
+    if (mixed) count ? launch_viss<256, 32, true, true>(a, nb, c->st) : launch_viss<256, 32, true, false>(a, nb, c->st);
+    else count ? launch_viss<256, 16, false, true>(a, nb, c->st) : launch_viss<256, 16, false, false>(a, nb, c->st);
+  } else {
+    if (mixed) count ? launch_viss<512, 32, true, true>(a, nb, c->st) : launch_viss<512, 32, true, false>(a, nb, c->st);
+    else count ? launch_viss<512, 16, false, true>(a, nb, c->st) : launch_viss<512, 16, false, false>(a, nb, c->st);
+  }
+  KC();
+  ev_end(c, ST_VISSK);
+  k_viss_reduce<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_part, s.nchunk, c->nh, c->nh_pad, c->d_vraw,
+                                                      count ? c->d_cpart : nullptr, count ? c->d_counts : nullptr);
+  KC();
+  ev_end(c, ST_VISS);
+  c->have_viss = true;
+  c->have_conj = c->have_groups = false;
+  return VW_OK;
+}
+
+int enq_conj(Ctx* c) {
+  if (!c->have_viss) return c->fail(VW_ERR_STATE, "vw_conj_fill needs vw_viss_half");
+  ev_begin(c, ST_CONJ);
+  k_conj_fill<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_vraw, c->d_rows, c->nh, c->P, c->d_vh, c->d_vfull);
+  KC();
+  ev_end(c, ST_CONJ);
+  c->have_conj = true;
+  c->have_groups = false;
+  return VW_OK;
+}
+
+int enq_group(Ctx* c) {
+  if (!c->have_conj) return c->fail(VW_ERR_STATE, "vw_recon_segment needs vw_conj_fill");
+  bool full = c->cfg.recon_mode == VW_RECON_FULL;
+  int64_t nr = full ? 2 * c->nh : c->nh;
+  ev_begin(c, ST_GROUP);
+  k_segparams<<<1, 256, 0, c->st>>>(c->d_rows, c->nh, c->P, c->bl_max, c->lambda, c->d_info);
+  KC();
+  k_keys<<<blocks_for(nr), 256, 0, c->st>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
+  KC();
+  k_rank_scatter<<<blocks_for(nr), 256, 0, c->st>>>(c->d_keys, nr, c->d_idx_sorted, c->d_keys_sorted);
+  KC();
+  k_groups<<<1, 1024, 0, c->st>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
+  KC();
+  k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->P, full ? 1 : 0,
+                                                  c->d_info, c->d_grec);
+  KC();
+  ev_end(c, ST_GROUP);
+  c->have_groups = true;
+  return VW_OK;
+}
+
+template <int PPL, bool MIXED, int KIND>
+void launch_recon(const ReconArgs& a, hipStream_t st) {
+  int64_t waves = (a.npts + 64 * PPL - 1) / (64 * PPL);
+  k_recon<PPL, MIXED, KIND><<<blocks_for(waves, 4), 256, 0, st>>>(a);
+}
+
+int enq_recon_set(Ctx* c, bool img) {
+  ReconArgs a{};
+  a.l = img ? c->d_gl : c->d_l;
+  a.m = img ? c->d_gm : c->d_m;
+  a.n = img ? c->d_gn : c->d_n;
+  a.npts = img ? c->ngp : c->npix;
+  a.TP = c->TP;
+  a.cls = (const uint32_t*)(img ? c->d_gcls : c->d_cls);
+  a.nh_pad = c->nh_pad;
+  a.rows = c->d_rows;
+  a.ant8 = c->d_ant8;
+  a.grec = c->d_grec;
+  a.idx_sorted = c->d_idx_sorted;
+  a.vh = c->d_vh;
+  a.info = c->d_info;
+  a.P = c->P;
+  a.full = c->cfg.recon_mode == VW_RECON_FULL;
+  a.occ_mode = c->cfg.occ_mode;
+  a.acc = img ? c->d_Gacc : c->d_Cacc;
+  a.wacc = img ? c->d_GWacc : c->d_Wacc;
+  int kind;
+  if (!c->cfg.blockage) kind = VW_KIND_AVG;
+  else if (!a.full && c->cfg.occ_mode == 2) kind = VW_KIND_REP;
+  else kind = VW_KIND_GENERIC;
+  bool mixed = c->cfg.precision == VW_PREC_MIXED;
+  ev_begin(c, ST_RECONK);
+  if (mixed) {
+    if (kind == VW_KIND_REP) launch_recon<1, true, VW_KIND_REP>(a, c->st);
+    else if (kind == VW_KIND_GENERIC) launch_recon<1, true, VW_KIND_GENERIC>(a, c->st);
+    else launch_recon<1, true, VW_KIND_AVG>(a, c->st);
+  } else {
+    if (kind == VW_KIND_REP) launch_recon<1, false, VW_KIND_REP>(a, c->st);
+    else if (kind == VW_KIND_GENERIC) launch_recon<1, false, VW_KIND_GENERIC>(a, c->st);
+    else launch_recon<1, false, VW_KIND_AVG>(a, c->st);
+  }
+  KC();
+  ev_end(c, ST_RECONK);
+  return VW_OK;
+}
+
+int enq_recon(Ctx* c) {
+  int rc;
+  if ((rc = enq_group(c))) return rc;
+  ev_begin(c, ST_RECON);
+  if (c->cfg.healpix_out && (rc = enq_recon_set(c, false))) return rc;
+  if (c->ngp && (rc = enq_recon_set(c, true))) return rc;
+  ev_end(c, ST_RECON);
+  return VW_OK;
+}
+
+int read_seg_info(Ctx* c, vw_seg_info* o) {
+  DevInfo di;
+  HC(hipMemcpy(&di, c->d_info, sizeof di, hipMemcpyDeviceToHost));
+  if (!o) return VW_OK;
+  o->t0 = c->t0;
+  o->tlen = c->tlen;
+  o->n_half = c->nh;
+  o->fa = di.sp.fa;
+  o->fb = di.sp.fb;
+  o->denom = di.sp.denom;
+  o->res = di.sp.res;
+  o->half = di.sp.half;
+  o->du = di.sp.du;
+  o->nuniq = di.nuniq;
+  o->nkeys = di.nuniq - di.first_nz;
+  return VW_OK;
+}
+
+Ctx* as_ctx(void* p) { return (Ctx*)p; }
+
+}  // namespace
+
+/* ================================================================== */
+/* C-ABI                                                              */
+/* ================================================================== */
+
+VW_API const char* vw_version(void) { return "vissws-gfx950 0.1"; }
+
+VW_API int vw_config_init(vw_config* cfg, int32_t geometry) {
+  if (!cfg) return VW_ERR_ARG;
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->geometry = geometry;
+  cfg->orbit_seed = 42;
+  cfg->day = 1;
+  cfg->blockage = 1;
+  cfg->occ_mode = 2;
+  cfg->recon_mode = VW_RECON_MIRROR;
+  cfg->tile_pix = 256;
+  cfg->precision = VW_PREC_PLAIN;
+  cfg->healpix_out = 1;
+  cfg->gm_km3_s2 = 398600.4418;
+  if (geometry == VW_GEOM_LUNAR8) {
+    cfg->nside = 4096;
+    cfg->freq_hz = 1e7;
+    cfg->n_ant = 8;
+    cfg->segs = 10;
+    cfg->occ_radius_km = VW_LUNAR_R_KM;
+    cfg->occ_altitude_km = VW_LUNAR_H_KM;
+    cfg->r_min_km = cfg->r_max_km = VW_LUNAR_A_KM;
+  } else if (geometry == VW_GEOM_CIRCULAR) {
+    cfg->nside = 32;
+    cfg->freq_hz = 1e7;
+    cfg->n_ant = 8;
+    cfg->n_epochs = 64;
+    cfg->dt_s = 10.0;
+    cfg->segs = 4;
+    cfg->occ_radius_km = 6371.0;
+    cfg->occ_altitude_km = -1.0;
+    cfg->r_min_km = 6900.0;
+    cfg->r_max_km = 7100.0;
+    cfg->img_n = 256;
+  } else {
+    return VW_ERR_ARG;
+  }
+  return VW_OK;
+}
+
+VW_API int vw_create(const vw_config* cfg, void** out) {
+  if (!out || !cfg) {
+    g_create_err = "null argument";
+    return VW_ERR_ARG;
+  }
+  *out = nullptr;
+  Ctx* c = new Ctx();
+  int rc = make_plan(c, cfg);
+  if (!rc) rc = create_device(c);
+  if (rc) {
+    g_create_err = c->err;
+    vw_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return VW_OK;
+}
+
+VW_API void vw_destroy(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void* q : c->allocs) (void)hipFree(q);
+  for (int s = 0; s < ST_N; ++s)
+    for (auto& e : c->ev[s]) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+VW_API const char* vw_last_error(void* p) {
+  if (!p) return g_create_err.c_str();
+  return as_ctx(p)->err.c_str();
+}
+
+VW_API int vw_get_plan(void* p, vw_plan* o) {
+  Ctx* c = as_ctx(p);
+  if (!c || !o) return VW_ERR_ARG;
+  std::memset(o, 0, sizeof *o);
+  o->npix = c->npix;
+  o->ntile = c->ntile;
+  o->n_img_pts = c->ngp;
+  o->n_img_tiles = c->ngt;
+  o->n_pairs = c->P;
+  o->n_epochs = c->T;
+  o->segs = c->segs;
+  o->max_tlen = c->max_tlen;
+  o->orbit_res = c->orbit_res;
+  o->seg_len = c->seg_len;
+  o->lambda_m = c->lambda;
+  o->bl_max_m = c->bl_max;
+  o->cosphi = c->cosphi_glob;
+  o->pix_area = c->pix_area;
+  return VW_OK;
+}
+
+VW_API int vw_get_schedule(void* p, int32_t* t0, int32_t* tl) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (t0) std::memcpy(t0, c->seg_t0.data(), sizeof(int32_t) * c->segs);
+  if (tl) std::memcpy(tl, c->seg_tlen.data(), sizeof(int32_t) * c->segs);
+  return VW_OK;
+}
+
+VW_API int vw_set_day(void* p, int32_t day) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  c->cfg.day = day;
+  int rc = upload_day(c);
+  if (rc) return rc;
+  c->have_seg = c->have_viss = c->have_conj = c->have_groups = false;
+  return sync(c);
+}
+
+VW_API int vw_set_sky(void* p, const double* B, int64_t npix, int32_t nsky) {
+  Ctx* c = as_ctx(p);
+  if (!c || !B) return VW_ERR_ARG;
+  if (npix != c->npix) return c->fail(VW_ERR_ARG, "sky has " + std::to_string(npix) + " pixels, expected " + std::to_string(c->npix));
+  if (nsky != 1) return c->fail(VW_ERR_ARG, "nsky must be 1");
+  std::vector<double> b((size_t)npix);
+  std::vector<float> bf((size_t)npix);
+  for (int64_t i = 0; i < npix; ++i) {
+    b[i] = B[i] * c->pix_area; /* main_recon_online.cu:106-110 in FP64 */
+    bf[i] = (float)b[i];
+  }
+  HC(hipMemcpyAsync(c->d_B, b.data(), sizeof(double) * npix, hipMemcpyHostToDevice, c->st));
+  HC(hipMemcpyAsync(c->d_Bf, bf.data(), sizeof(float) * npix, hipMemcpyHostToDevice, c->st));
+  HC(hipStreamSynchronize(c->st));
+  c->have_sky = true;
+  return VW_OK;
+}
+
+VW_API int vw_gen_segment(void* p, int32_t t0, int32_t tlen, vw_geom_host* o) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  int rc = enq_gen(c, t0, tlen);
+  if (rc) return rc;
+  if (o) {
+    if (!c->d_full && (rc = dalloc(c, &c->d_full, 9 * 2 * (size_t)c->max_nh))) return rc;
+    int64_t nf = 2 * c->nh;
+    k_seg_full<<<blocks_for(nf), 256, 0, c->st>>>(c->d_pos, c->T, c->A, c->P, t0, c->nh, c->d_pa, c->d_pb, c->d_rows,
+                                                  c->d_full);
+    KC();
+    double* dst[9] = {o->u, o->v, o->w, o->xa, o->ya, o->za, o->xb, o->yb, o->zb};
+    for (int k = 0; k < 9; ++k)
+      if (dst[k]) HC(hipMemcpyAsync(dst[k], c->d_full + k * nf, sizeof(double) * nf, hipMemcpyDeviceToHost, c->st));
+    HC(hipStreamSynchronize(c->st));
+    /* half-symmetry guard (orbit_online.cuh:403-418, common.hpp:222-226) */
+    if (o->u && o->v && o->w) {
+      for (int64_t g = 0; g < c->tlen; ++g)
+        for (int j = 0; j < c->P; ++j) {
+          int64_t i0 = g * 2 * c->P + j, i1 = i0 + c->P;
+          const double* arr[3] = {o->u, o->v, o->w};
+          for (int k = 0; k < 3; ++k) {
+            double a = arr[k][i0], b = arr[k][i1];
+            double tol = 1e-5 + 1e-5 * std::fmax(std::fabs(a), std::fabs(b));
+            if (!(std::fabs(a + b) <= tol)) return c->fail(VW_ERR_HALFSYM, "generated segment uvw violates half symmetry");
+          }
+        }
+    }
+  }
+  return sync(c);
+}
+
+VW_API int vw_viss_half(void* p, vw_c128* out) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  int rc = enq_viss(c, false);
+  if (rc) return rc;
+  if (out) HC(hipMemcpyAsync(out, c->d_vraw, sizeof(vw_c128) * c->nh, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+VW_API int vw_conj_fill(void* p, vw_c128* oh, vw_c128* of) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  int rc = enq_conj(c);
+  if (rc) return rc;
+  if (oh) HC(hipMemcpyAsync(oh, c->d_vh, sizeof(vw_c128) * c->nh, hipMemcpyDeviceToHost, c->st));
+  if (of) HC(hipMemcpyAsync(of, c->d_vfull, sizeof(vw_c128) * 2 * c->nh, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+VW_API int vw_recon_segment(void* p, vw_seg_info* info) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  int rc = enq_recon(c);
+  if (rc) return rc;
+  if ((rc = sync(c))) return rc;
+  return read_seg_info(c, info);
+}
+
+VW_API int vw_run_all(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  int rc;
+  for (int s = 0; s < c->segs; ++s) {
+    if ((rc = enq_gen(c, c->seg_t0[s], c->seg_tlen[s])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c)) ||
+        (rc = enq_recon(c)))
+      return rc;
+  }
+  return sync(c);
+}
+
+VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* Wimg) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  ev_begin(c, ST_FIN);
+  /* normalise on the device; copy out only what was asked for */
+  if (c->cfg.healpix_out) {
+    k_normalize<<<blocks_for(c->npix), 256, 0, c->st>>>(c->d_Cacc, c->d_Wacc, c->npix, c->cfg.blockage, c->d_info,
+                                                        nullptr, c->d_out, c->d_wout);
+    KC();
+  }
+  if (c->ngp) {
+    int64_t NN = (int64_t)c->cfg.img_n * c->cfg.img_n;
+    HC(hipMemsetAsync(c->d_iout, 0, sizeof(double) * NN, c->st));
+    HC(hipMemsetAsync(c->d_iwout, 0, sizeof(uint32_t) * NN, c->st));
+    k_normalize<<<blocks_for(c->ngp), 256, 0, c->st>>>(c->d_Gacc, c->d_GWacc, c->ngp, c->cfg.blockage, c->d_info,
+                                                       c->d_gidx, c->d_iout, c->d_iwout);
+    KC();
+  }
+  ev_end(c, ST_FIN);
+  if (C) {
+    if (c->cfg.healpix_out) HC(hipMemcpyAsync(C, c->d_out, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
+    else std::memset(C, 0, sizeof(double) * c->npix);
+  }
+  if (W) {
+    if (c->cfg.healpix_out) HC(hipMemcpyAsync(W, c->d_wout, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
+    else std::memset(W, 0, sizeof(uint32_t) * c->npix);
+  }
+  if (c->ngp) {
+    int64_t NN = (int64_t)c->cfg.img_n * c->cfg.img_n;
+    if (img) HC(hipMemcpyAsync(img, c->d_iout, sizeof(double) * NN, hipMemcpyDeviceToHost, c->st));
+    if (Wimg) HC(hipMemcpyAsync(Wimg, c->d_iwout, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost, c->st));
+  }
+  return sync(c);
+}
+
+VW_API int vw_reset_accumulators(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
+  HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * c->npix, c->st));
+  if (c->ngp) {
+    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
+  }
+  HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
+  return sync(c);
+}
+
+VW_API int vw_get_timings(void* p, vw_timings* o) {
+  Ctx* c = as_ctx(p);
+  if (!c || !o) return VW_ERR_ARG;
+  *o = c->tim;
+  c->tim = vw_timings{};
+  return VW_OK;
+}
+
+VW_API int vw_get_lmn(void* p, double* l, double* m, double* n, double* nm1) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  size_t b = sizeof(double) * c->npix;
+  if (l) HC(hipMemcpy(l, c->d_l, b, hipMemcpyDeviceToHost));
+  if (m) HC(hipMemcpy(m, c->d_m, b, hipMemcpyDeviceToHost));
+  if (n) HC(hipMemcpy(n, c->d_n, b, hipMemcpyDeviceToHost));
+  if (nm1) HC(hipMemcpy(nm1, c->d_nm1, b, hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_get_tiles(void* p, int32_t img, double* cone5) {
+  Ctx* c = as_ctx(p);
+  if (!c || !cone5) return VW_ERR_ARG;
+  if (img && !c->ngp) return c->fail(VW_ERR_ARG, "no lm grid");
+  HC(hipMemcpy(cone5, img ? c->d_gcone : c->d_cone, sizeof(double) * 5 * (img ? c->ngt : c->ntile), hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_get_img_points(void* p, int64_t* idx, double* l, double* m, double* n) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->ngp) return c->fail(VW_ERR_ARG, "no lm grid");
+  size_t b = sizeof(double) * c->ngp;
+  if (idx) std::memcpy(idx, c->gidx.data(), sizeof(int64_t) * c->ngp);
+  if (l) HC(hipMemcpy(l, c->d_gl, b, hipMemcpyDeviceToHost));
+  if (m) HC(hipMemcpy(m, c->d_gm, b, hipMemcpyDeviceToHost));
+  if (n) HC(hipMemcpy(n, c->d_gn, b, hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_get_positions(void* p, double* pos) {
+  Ctx* c = as_ctx(p);
+  if (!c || !pos) return VW_ERR_ARG;
+  HC(hipMemcpy(pos, c->d_pos, sizeof(double) * 3 * c->T * c->A, hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_get_rows(void* p, double* u, double* v, double* w) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->have_seg) return c->fail(VW_ERR_STATE, "no segment");
+  std::vector<RowRec> r((size_t)c->nh);
+  HC(hipMemcpy(r.data(), c->d_rows, sizeof(RowRec) * c->nh, hipMemcpyDeviceToHost));
+  for (int64_t h = 0; h < c->nh; ++h) {
+    if (u) u[h] = r[h].u;
+    if (v) v[h] = r[h].v;
+    if (w) w[h] = r[h].w;
+  }
+  return VW_OK;
+}
+
+VW_API int vw_get_tile_classes(void* p, int32_t img, uint8_t* cls) {
+  Ctx* c = as_ctx(p);
+  if (!c || !cls) return VW_ERR_ARG;
+  if (!c->have_seg) return c->fail(VW_ERR_STATE, "no segment");
+  if (img && !c->ngp) return c->fail(VW_ERR_ARG, "no lm grid");
+  int64_t nt = img ? c->ngt : c->ntile;
+  std::vector<uint8_t> t((size_t)(nt * c->nh_pad));
+  HC(hipMemcpy(t.data(), img ? c->d_gcls : c->d_cls, t.size(), hipMemcpyDeviceToHost));
+  for (int64_t h = 0; h < c->nh; ++h)
+    for (int64_t k = 0; k < nt; ++k) cls[h * nt + k] = t[k * c->nh_pad + h];
+  return VW_OK;
+}
+
+VW_API int vw_get_masks(void* p, int64_t row0, int64_t nrows, uint8_t* mask) {
+  Ctx* c = as_ctx(p);
+  if (!c || !mask) return VW_ERR_ARG;
+  if (!c->have_seg) return c->fail(VW_ERR_STATE, "no segment");
+  if (row0 < 0 || nrows < 0 || row0 + nrows > c->nh) return c->fail(VW_ERR_ARG, "row range");
+  int64_t tot = nrows * c->npix;
+  uint8_t* d = nullptr;
+  HC(hipMalloc(&d, tot ? tot : 1));
+  k_masks<<<blocks_for(tot), 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->TP, c->d_cls, c->nh_pad, c->d_rows,
+                                              c->d_ant8, row0, nrows, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(mask, d, tot, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("vw_get_masks: ") + hipGetErrorString(e));
+  return VW_OK;
+}
+
+VW_API int vw_get_viss_counts(void* p, int64_t* counts) {
+  Ctx* c = as_ctx(p);
+  if (!c || !counts) return VW_ERR_ARG;
+  int rc = enq_viss(c, true);
+  if (rc) return rc;
+  HC(hipMemcpyAsync(counts, c->d_counts, sizeof(int64_t) * c->nh, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+VW_API int vw_get_groups(void* p, vw_seg_info* info, int64_t* keys, int32_t* offsets, int32_t* idx_sorted,
+                         int32_t* rep) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->have_groups) return c->fail(VW_ERR_STATE, "no groups (call vw_recon_segment)");
+  vw_seg_info si;
+  int rc = read_seg_info(c, &si);
+  if (rc) return rc;
+  if (info) *info = si;
+  int64_t nr = c->cfg.recon_mode == VW_RECON_FULL ? 2 * c->nh : c->nh;
+  if (keys) HC(hipMemcpy(keys, c->d_gkeys, sizeof(int64_t) * si.nuniq, hipMemcpyDeviceToHost));
+  if (offsets) HC(hipMemcpy(offsets, c->d_goff, sizeof(int32_t) * (si.nuniq + 1), hipMemcpyDeviceToHost));
+  if (idx_sorted) HC(hipMemcpy(idx_sorted, c->d_idx_sorted, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+  if (rep) {
+    std::vector<int32_t> srt((size_t)nr), off((size_t)si.nuniq + 1);
+    HC(hipMemcpy(srt.data(), c->d_idx_sorted, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(off.data(), c->d_goff, sizeof(int32_t) * (si.nuniq + 1), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < si.nuniq; ++q) rep[q] = srt[off[q] + (off[q + 1] - off[q]) / 2];
+  }
+  return VW_OK;
+}

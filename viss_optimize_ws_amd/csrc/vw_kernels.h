@@ -1,0 +1,792 @@
+/*
+ * vw_kernels.h -- gfx950 kernels of the visibility / reconstruction path.
+ *
+ * Execution model (wave64, CDNA4):
+ *  - Viss: a wave owns one chunk of sky tiles x RPW half rows.  Lanes hold
+ *    pixels (TP/64 per lane, coalesced SoA loads), rows are wave-uniform:
+ *    row records, tile-cone classes and antenna frames arrive through scalar
+ *    loads, so the all-hidden / all-visible / mixed decision is a scalar
+ *    branch with no divergence.  Each lane keeps RPW complex partial sums in
+ *    registers; one cross-lane tree per row per chunk writes a partial that
+ *    a second kernel folds in chunk order (deterministic).
+ *  - Recon: lanes hold output points (one tile per wave group), groups are
+ *    wave-uniform; the plain form sums groups in ascending order per point,
+ *    exactly as the oracle does.
+ * Workgroups are remapped so that consecutive logical blocks land on the same
+ * XCD (hardware dispatches block b to XCD b % 8), keeping a chunk's pixel
+ * data in one XCD's L2.
+ */
+#ifndef VW_KERNELS_H
+#define VW_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "vw_shared.h"
+#include "../../include/vissws.h"
+
+struct RowRec {
+  double u, v, w;
+  int32_t ia, ib; /* antenna-frame indices g*A + a of the two endpoints */
+};
+
+struct GroupRec {
+  double ugu, vgu, vr, vi; /* cell centre (wavelengths) and visibility   */
+  int32_t rep;             /* representative half row (mirror occ 2)     */
+  int32_t s0, s1;          /* member range in idx_sorted                 */
+  int32_t key_nz;          /* key != 0                                   */
+};
+
+struct DevInfo {
+  vw_segparams sp;
+  int64_t nuniq;
+  int64_t first_nz;
+  int64_t wtotal;
+  int64_t pad;
+};
+
+#define VW_FULLMASK 0xffffffffffffffffULL
+
+__device__ __forceinline__ int vw_rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+/* logical block index with consecutive indices on one XCD */
+__device__ __forceinline__ int64_t vw_xcd_block(int64_t b, int64_t nb) {
+  int64_t q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ double vw_wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ int64_t vw_wave_sum_i(int64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+/* class byte of (tile, half row) from a [ntile][nh_pad] byte table, uniform */
+__device__ __forceinline__ int vw_cls_lookup(const uint32_t* __restrict__ words, int64_t tile,
+                                             int64_t nh_pad, int64_t h) {
+  int64_t byte = tile * nh_pad + h;
+  uint32_t w = words[byte >> 2];
+  return vw_rfl((int)((w >> (8 * (byte & 3))) & 0xffu));
+}
+
+/* ------------------------------------------------------------------ */
+/* resident geometry                                                    */
+/* ------------------------------------------------------------------ */
+__global__ void k_pix2lmn(int64_t nside, int64_t npix, double* __restrict__ l, double* __restrict__ m,
+                          double* __restrict__ n, double* __restrict__ nm1) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  double a, b, c;
+  vw_pix2lmn_nest(nside, p, &a, &b, &c);
+  l[p] = a;
+  m[p] = b;
+  n[p] = c;
+  nm1[p] = c - 1.0;
+}
+
+__global__ void k_grid_points(int64_t N, int64_t npts, const int64_t* __restrict__ gidx,
+                              double* __restrict__ l, double* __restrict__ m, double* __restrict__ n) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  int64_t id = gidx[k];
+  double a, b, c;
+  vw_grid_point(N, id / N, id % N, &a, &b, &c);
+  l[k] = a;
+  m[k] = b;
+  n[k] = c;
+}
+
+/* Tile cone: stride-halving pairwise sum (a[i] += a[i+s], s = TP/2..1), centre,
+ * minimum of c.s over the tile.  One workgroup of 256 per tile. */
+template <int TP>
+__global__ __launch_bounds__(256) void k_tile_cone(const double* __restrict__ l, const double* __restrict__ m,
+                                                   const double* __restrict__ n, int64_t npts, int64_t ntile,
+                                                   double* __restrict__ cone) {
+  __shared__ double sx[TP / 2], sy[TP / 2], sz[TP / 2];
+  __shared__ double smin[256];
+  int64_t t = blockIdx.x;
+  int64_t p0 = t * TP;
+  int64_t cnt = npts - p0 < TP ? npts - p0 : TP;
+  for (int i = threadIdx.x; i < TP / 2; i += 256) {
+    int i2 = i + TP / 2;
+    double x0 = i < cnt ? l[p0 + i] : 0.0, x1 = i2 < cnt ? l[p0 + i2] : 0.0;
+    double y0 = i < cnt ? m[p0 + i] : 0.0, y1 = i2 < cnt ? m[p0 + i2] : 0.0;
+    double z0 = i < cnt ? n[p0 + i] : 0.0, z1 = i2 < cnt ? n[p0 + i2] : 0.0;
+    sx[i] = x0 + x1;
+    sy[i] = y0 + y1;
+    sz[i] = z0 + z1;
+  }
+  __syncthreads();
+  for (int s = TP / 4; s > 0; s >>= 1) {
+    for (int i = threadIdx.x; i < s; i += 256) {
+      sx[i] = sx[i] + sx[i + s];
+      sy[i] = sy[i] + sy[i + s];
+      sz[i] = sz[i] + sz[i + s];
+    }
+    __syncthreads();
+  }
+  double cx, cy, cz;
+  vw_cone_centre(sx[0], sy[0], sz[0], &cx, &cy, &cz);
+  double mind = 1.0;
+  for (int64_t i = threadIdx.x; i < cnt; i += 256)
+    mind = fmin(mind, vw_dot3(cx, cy, cz, l[p0 + i], m[p0 + i], n[p0 + i]));
+  smin[threadIdx.x] = mind;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double ca, sa;
+    vw_cone_radius(smin[0], &ca, &sa);
+    cone[0 * ntile + t] = cx;
+    cone[1 * ntile + t] = cy;
+    cone[2 * ntile + t] = cz;
+    cone[3 * ntile + t] = ca;
+    cone[4 * ntile + t] = sa;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* day geometry and segment rows                                        */
+/* ------------------------------------------------------------------ */
+struct DayArgs {
+  int geometry;
+  int T, A;
+  int64_t orbit_res, seg_len, tw, day_offset;
+  int day;
+  const int64_t* t1s;
+  const double* circ; /* 7*A: radius, phase, rate, sr, cr, si, ci */
+  double dt;
+};
+
+__global__ void k_day_positions(DayArgs a, double* __restrict__ pos) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t TA = (int64_t)a.T * a.A;
+  if (k >= TA) return;
+  int64_t t = k / a.A;
+  int s = (int)(k - t * a.A);
+  double x, y, z;
+  if (a.geometry == VW_GEOM_LUNAR8) {
+    int64_t orb = t / a.seg_len, loc = t - orb * a.seg_len;
+    int64_t row = a.t1s[orb] + loc;
+    vw_lunar_sat(row, s, a.orbit_res, a.day, a.tw, a.day_offset, &x, &y, &z);
+  } else {
+    const double* c = a.circ;
+    int A = a.A;
+    vw_circ_orbit(c[s], c[A + s], c[2 * A + s], (double)t * a.dt, c[3 * A + s], c[4 * A + s],
+                  c[5 * A + s], c[6 * A + s], &x, &y, &z);
+  }
+  pos[k] = x;
+  pos[TA + k] = y;
+  pos[2 * TA + k] = z;
+}
+
+/* per (epoch, antenna) frame: px,py,pz,invn,ux,uy,uz,cosphi */
+__global__ void k_seg_ant(const double* __restrict__ pos, int T, int A, int t0, int tlen, double occ_r,
+                          double cosphi_glob, double* __restrict__ ant8) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)tlen * A) return;
+  int64_t TA = (int64_t)T * A;
+  int64_t src = (int64_t)t0 * A + k;
+  double px = pos[src], py = pos[TA + src], pz = pos[2 * TA + src];
+  double in, ux, uy, uz, cp;
+  vw_antenna_frame(px, py, pz, occ_r, cosphi_glob, &in, &ux, &uy, &uz, &cp);
+  double* q = ant8 + k * 8;
+  q[0] = px; q[1] = py; q[2] = pz; q[3] = in;
+  q[4] = ux; q[5] = uy; q[6] = uz; q[7] = cp;
+}
+
+__global__ void k_seg_rows(const double* __restrict__ pos, int T, int A, int P, int t0, int64_t nh,
+                           const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, double lambda,
+                           RowRec* __restrict__ rows) {
+  int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  int64_t TA = (int64_t)T * A;
+  int64_t g = h / P;
+  int j = (int)(h - g * P);
+  int64_t ka = (t0 + g) * A + pa[j], kb = (t0 + g) * A + pb[j];
+  RowRec r;
+  r.u = vw_uvw_comp(pos[ka], pos[kb], lambda);
+  r.v = vw_uvw_comp(pos[TA + ka], pos[TA + kb], lambda);
+  r.w = vw_uvw_comp(pos[2 * TA + ka], pos[2 * TA + kb], lambda);
+  r.ia = (int32_t)(g * A + pa[j]);
+  r.ib = (int32_t)(g * A + pb[j]);
+  rows[h] = r;
+}
+
+/* reference full-row layout (orbit_online.cuh:186-231) into 9 SoA arrays */
+__global__ void k_seg_full(const double* __restrict__ pos, int T, int A, int P, int t0, int64_t nh,
+                           const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                           const RowRec* __restrict__ rows, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t nf = 2 * nh;
+  if (i >= nf) return;
+  int64_t TA = (int64_t)T * A;
+  int64_t g = i / (2 * P);
+  int jj = (int)(i - g * 2 * P);
+  bool neg = jj >= P;
+  int j = neg ? jj - P : jj;
+  int64_t h = g * P + j;
+  int64_t ka = (t0 + g) * A + pa[j], kb = (t0 + g) * A + pb[j];
+  int64_t k1 = neg ? kb : ka, k2 = neg ? ka : kb;
+  RowRec r = rows[h];
+  out[0 * nf + i] = neg ? -r.u : r.u;
+  out[1 * nf + i] = neg ? -r.v : r.v;
+  out[2 * nf + i] = neg ? -r.w : r.w;
+  out[3 * nf + i] = pos[k1];
+  out[4 * nf + i] = pos[TA + k1];
+  out[5 * nf + i] = pos[2 * TA + k1];
+  out[6 * nf + i] = pos[k2];
+  out[7 * nf + i] = pos[TA + k2];
+  out[8 * nf + i] = pos[2 * TA + k2];
+}
+
+/* pair class per (tile, half row) -> bytes [ntile][nh_pad] */
+__global__ void k_classify(const double* __restrict__ cone, int64_t ntile, const RowRec* __restrict__ rows,
+                           const double* __restrict__ ant8, int64_t nh, int64_t nh_pad,
+                           uint8_t* __restrict__ cls) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = ntile * nh_pad;
+  if (k >= total) return;
+  int64_t t = k / nh_pad, h = k - t * nh_pad;
+  if (h >= nh) { cls[k] = VW_CLS_HIDDEN; return; }
+  RowRec r = rows[h];
+  const double* qa = ant8 + (int64_t)r.ia * 8;
+  const double* qb = ant8 + (int64_t)r.ib * 8;
+  double cx = cone[t], cy = cone[ntile + t], cz = cone[2 * ntile + t];
+  double ca = cone[3 * ntile + t], sa = cone[4 * ntile + t];
+  int ba = vw_cone_bits(qa[4], qa[5], qa[6], cx, cy, cz, ca, sa, qa[7]);
+  int bb = vw_cone_bits(qb[4], qb[5], qb[6], cx, cy, cz, ca, sa, qb[7]);
+  cls[k] = (uint8_t)vw_pair_class(ba, bb);
+}
+
+/* effective mask of rows [row0, row0+nrows) x pixels (introspection) */
+__global__ void k_masks(const double* __restrict__ l, const double* __restrict__ m, const double* __restrict__ n,
+                        int64_t npix, int64_t TP, const uint8_t* __restrict__ cls, int64_t nh_pad,
+                        const RowRec* __restrict__ rows, const double* __restrict__ ant8, int64_t row0,
+                        int64_t nrows, uint8_t* __restrict__ mask) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows * npix) return;
+  int64_t r = k / npix, p = k - r * npix;
+  int64_t h = row0 + r;
+  int c = cls[(p / TP) * nh_pad + h];
+  uint8_t v;
+  if (c == VW_CLS_VISIBLE) v = 1;
+  else if (c == VW_CLS_HIDDEN) v = 0;
+  else {
+    RowRec rr = rows[h];
+    const double* qa = ant8 + (int64_t)rr.ia * 8;
+    const double* qb = ant8 + (int64_t)rr.ib * 8;
+    v = (uint8_t)(vw_visible(l[p], m[p], n[p], qa[0], qa[1], qa[2], qa[3], qa[7]) &&
+                  vw_visible(l[p], m[p], n[p], qb[0], qb[1], qb[2], qb[3], qb[7]));
+  }
+  mask[k] = v;
+}
+
+/* ------------------------------------------------------------------ */
+/* Viss                                                                 */
+/* ------------------------------------------------------------------ */
+struct VissArgs {
+  const double *l, *m, *n, *nm1, *B;
+  const float* Bf;
+  int64_t npix, ntile;
+  const RowRec* rows;
+  const double* ant8;
+  const uint32_t* cls; /* [ntile][nh_pad] bytes viewed as words */
+  int64_t nh, nh_pad;
+  int64_t tpc, nchunk, nrb, bpc;
+  double2* part; /* [nchunk][nh_pad] */
+  int64_t* cpart; /* [nchunk][nh_pad] (counting instance) */
+};
+
+__device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
+  *s = __builtin_amdgcn_sinf(r);
+  *c = __builtin_amdgcn_cosf(r);
+}
+
+template <int TP, int RPW, bool MIXED, bool COUNT>
+__global__ __launch_bounds__(256) void k_viss(VissArgs a) {
+  constexpr int PPL = TP / 64;
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t chunk = vb / a.bpc;
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int64_t rb = (vb - chunk * a.bpc) * 4 + wave;
+  if (chunk >= a.nchunk || rb >= a.nrb) return;
+  const int64_t r0 = rb * RPW;
+  const int64_t t0 = chunk * a.tpc;
+  const int64_t t1 = (t0 + a.tpc < a.ntile) ? t0 + a.tpc : a.ntile;
+
+  float fre[RPW], fim[RPW];
+  double dre[RPW], dim[RPW];
+  int cnt[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) { fre[r] = 0.f; fim[r] = 0.f; dre[r] = 0.0; dim[r] = 0.0; cnt[r] = 0; }
+
+  for (int64_t tile = t0; tile < t1; ++tile) {
+    double pl[PPL], pm[PPL], pn[PPL], pz[PPL], pb[PPL];
+    float pbf[PPL];
+    bool ok[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      int64_t p = tile * TP + lane + 64 * k;
+      ok[k] = p < a.npix;
+      int64_t pp = ok[k] ? p : 0;
+      pl[k] = a.l[pp];
+      pm[k] = a.m[pp];
+      pn[k] = a.n[pp];
+      pz[k] = a.nm1[pp];
+      if (MIXED) pbf[k] = ok[k] ? a.Bf[pp] : 0.0f;
+      else pb[k] = ok[k] ? a.B[pp] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int64_t h = r0 + r;
+      if (h < a.nh) {
+        const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, h);
+        if (cls != VW_CLS_HIDDEN) {
+          const RowRec rec = a.rows[h];
+          bool vis[PPL];
+          if (cls == VW_CLS_MIXED) {
+            const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+            const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+            const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
+            const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+              vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                       vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+          } else {
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
+          }
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) {
+            const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl[k], pm[k], pz[k]);
+            if (MIXED) {
+              const float f = (float)(x - rint(x));
+              float s, c;
+              vw_hw_sincos_turn(f, &s, &c);
+              const float bb = vis[k] ? pbf[k] : 0.0f;
+              fre[r] = fmaf(bb, c, fre[r]);
+              fim[r] = fmaf(bb, s, fim[r]);
+            } else {
+              double s, c;
+              vw_sincos_turn(x, &s, &c);
+              const double bb = vis[k] ? pb[k] : 0.0;
+              dre[r] = fma(bb, c, dre[r]);
+              dim[r] = fma(bb, s, dim[r]);
+            }
+            if (COUNT) cnt[r] += vis[k] ? 1 : 0;
+          }
+        }
+      }
+    }
+  }
+  /* one cross-lane tree per row; FP64 above the per-lane FP32 blocks */
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int64_t h = r0 + r;
+    double re = MIXED ? (double)fre[r] : dre[r];
+    double im = MIXED ? (double)fim[r] : dim[r];
+    re = vw_wave_sum(re);
+    im = vw_wave_sum(im);
+    int64_t cc = 0;
+    if (COUNT) cc = vw_wave_sum_i((int64_t)cnt[r]);
+    if (lane == 0 && h < a.nh) {
+      a.part[chunk * a.nh_pad + h] = make_double2(re, -im);
+      if (COUNT) a.cpart[chunk * a.nh_pad + h] = cc;
+    }
+  }
+}
+
+/* partials -> raw half sums, chunks folded in ascending order */
+__global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, int64_t nh, int64_t nh_pad,
+                              double2* __restrict__ vraw, const int64_t* __restrict__ cpart,
+                              int64_t* __restrict__ counts) {
+  int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  double re = 0.0, im = 0.0;
+  int64_t c = 0;
+  for (int64_t k = 0; k < nchunk; ++k) {
+    double2 v = part[k * nh_pad + h];
+    re += v.x;
+    im += v.y;
+    if (counts) c += cpart[k * nh_pad + h];
+  }
+  vraw[h] = make_double2(re, im);
+  if (counts) counts[h] = c;
+}
+
+/* phase correction + conjugate completion (common.hpp:255-277) */
+__global__ void k_conj_fill(const double2* __restrict__ vraw, const RowRec* __restrict__ rows, int64_t nh, int P,
+                            double2* __restrict__ vh, double2* __restrict__ vfull) {
+  int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  double2 z = vraw[h];
+  double re, im;
+  vw_phase_correct(z.x, z.y, rows[h].w, &re, &im);
+  vh[h] = make_double2(re, im);
+  int64_t g = h / P, j = h - g * P;
+  vfull[2 * P * g + j] = make_double2(re, im);
+  vfull[2 * P * g + P + j] = make_double2(re, -im);
+}
+
+/* ------------------------------------------------------------------ */
+/* segment parameters, keys, stable grouping                            */
+/* ------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_segparams(const RowRec* __restrict__ rows, int64_t nh, int P,
+                                                   double bl_max, double lambda, DevInfo* __restrict__ info) {
+  __shared__ double s[5][256];
+  const int t = threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+  for (int64_t i = t; i < 2 * nh; i += 256) {
+    int64_t g = i / (2 * P);
+    int jj = (int)(i - g * 2 * P);
+    bool neg = jj >= P;
+    int64_t h = g * P + (neg ? jj - P : jj);
+    double sg = neg ? -1.0 : 1.0;
+    RowRec r = rows[h];
+    double u = sg * r.u, v = sg * r.v, w = sg * r.w;
+    a0 = a0 + u * u;
+    a1 = a1 + v * v;
+    a2 = a2 + u * v;
+    a3 = a3 + u * w;
+    a4 = a4 + v * w;
+  }
+  s[0][t] = a0; s[1][t] = a1; s[2][t] = a2; s[3][t] = a3; s[4][t] = a4;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st)
+      for (int k = 0; k < 5; ++k) s[k][t] = s[k][t] + s[k][t + st];
+    __syncthreads();
+  }
+  if (t == 0) vw_segparams_from_sums(s[0][0], s[1][0], s[2][0], s[3][0], s[4][0], bl_max, lambda, &info->sp);
+}
+
+/* mode row -> (half row, conjugate flag) */
+__device__ __forceinline__ int64_t vw_row_half(int64_t r, int P, bool full, bool* neg) {
+  if (!full) { *neg = false; return r; }
+  int64_t g = r / (2 * P);
+  int jj = (int)(r - g * 2 * P);
+  *neg = jj >= P;
+  return g * P + (jj >= P ? jj - P : jj);
+}
+
+__global__ void k_keys(const RowRec* __restrict__ rows, int64_t nr, int P, int full, const DevInfo* __restrict__ info,
+                       int64_t* __restrict__ keys) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  bool neg;
+  int64_t h = vw_row_half(r, P, full != 0, &neg);
+  double sg = neg ? -1.0 : 1.0;
+  RowRec rec = rows[h];
+  vw_segparams sp = info->sp;
+  keys[r] = vw_grid_key(sg * rec.u, sg * rec.v, &sp, full ? 0 : 1);
+}
+
+/* stable rank by (key, index); all-pairs count with LDS-staged key tiles */
+__global__ __launch_bounds__(256) void k_rank_scatter(const int64_t* __restrict__ keys, int64_t nr,
+                                                      int32_t* __restrict__ idx_sorted,
+                                                      int64_t* __restrict__ keys_sorted) {
+  __shared__ int64_t sk[256];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t ki = i < nr ? keys[i] : 0;
+  int64_t rank = 0;
+  for (int64_t base = 0; base < nr; base += 256) {
+    int64_t j = base + threadIdx.x;
+    sk[threadIdx.x] = j < nr ? keys[j] : INT64_MAX;
+    __syncthreads();
+    int64_t lim = nr - base < 256 ? nr - base : 256;
+    for (int64_t q = 0; q < lim; ++q) {
+      int64_t kj = sk[q];
+      rank += (kj < ki) || (kj == ki && base + q < i);
+    }
+    __syncthreads();
+  }
+  if (i < nr) {
+    idx_sorted[rank] = (int32_t)i;
+    keys_sorted[rank] = ki;
+  }
+}
+
+/* run boundaries -> keys, offsets, nuniq (single workgroup) */
+__global__ __launch_bounds__(1024) void k_groups(const int64_t* __restrict__ ks, int64_t nr, int blockage,
+                                                 int64_t* __restrict__ gkeys, int32_t* __restrict__ goff,
+                                                 DevInfo* __restrict__ info) {
+  __shared__ int64_t sc[1024];
+  const int t = threadIdx.x;
+  int64_t per = (nr + 1023) / 1024;
+  int64_t b = t * per, e = b + per < nr ? b + per : nr;
+  int64_t c = 0;
+  for (int64_t k = b; k < e; ++k) c += (k == 0 || ks[k] != ks[k - 1]);
+  sc[t] = c;
+  __syncthreads();
+  for (int s = 1; s < 1024; s <<= 1) {
+    int64_t v = t >= s ? sc[t - s] : 0;
+    __syncthreads();
+    sc[t] += v;
+    __syncthreads();
+  }
+  int64_t g = sc[t] - c; /* exclusive */
+  for (int64_t k = b; k < e; ++k) {
+    if (k == 0 || ks[k] != ks[k - 1]) {
+      gkeys[g] = ks[k];
+      goff[g] = (int32_t)k;
+      ++g;
+    }
+  }
+  if (t == 1023) {
+    int64_t nu = sc[1023];
+    goff[nu] = (int32_t)nr;
+    info->nuniq = nu;
+    info->first_nz = (nu > 0 && ks[0] == 0) ? 1 : 0;
+    if (!blockage) info->wtotal += nu;
+  }
+}
+
+__global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* __restrict__ goff,
+                             const int32_t* __restrict__ idx_sorted, const double2* __restrict__ vh, int P, int full,
+                             const DevInfo* __restrict__ info, GroupRec* __restrict__ grec) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= info->nuniq) return;
+  vw_segparams sp = info->sp;
+  GroupRec g;
+  int64_t key = gkeys[q];
+  int32_t s0 = goff[q], s1 = goff[q + 1];
+  g.s0 = s0;
+  g.s1 = s1;
+  g.key_nz = key != 0;
+  g.ugu = 0.0;
+  g.vgu = 0.0;
+  if (key != 0) vw_key_uv(key, &sp, &g.ugu, &g.vgu);
+  bool neg;
+  int64_t hr = vw_row_half(idx_sorted[s0 + (s1 - s0) / 2], P, full != 0, &neg);
+  g.rep = (int32_t)hr;
+  /* group mean (blockage 0) in member order; representative value otherwise */
+  double sr = 0.0, si = 0.0;
+  for (int32_t k = s0; k < s1; ++k) {
+    bool ng;
+    int64_t h = vw_row_half(idx_sorted[k], P, full != 0, &ng);
+    double2 v = vh[h];
+    sr = sr + v.x;
+    si = si + (ng ? -v.y : v.y);
+  }
+  g.vr = sr / (double)(s1 - s0);
+  g.vi = si / (double)(s1 - s0);
+  grec[q] = g;
+}
+
+/* ------------------------------------------------------------------ */
+/* reconstruction                                                       */
+/* ------------------------------------------------------------------ */
+struct ReconArgs {
+  const double *l, *m, *n;
+  int64_t npts, TP;
+  const uint32_t* cls; /* [ntile_out][nh_pad] */
+  int64_t nh_pad;
+  const RowRec* rows;
+  const double* ant8;
+  const GroupRec* grec;
+  const int32_t* idx_sorted;
+  const double2* vh;
+  const DevInfo* info;
+  int P, full, occ_mode;
+  double* acc;   /* accumulated map (npts)        */
+  uint32_t* wacc; /* accumulated weights (npts)     */
+};
+
+#define VW_KIND_REP 0
+#define VW_KIND_GENERIC 1
+#define VW_KIND_AVG 2
+
+/* one row's effective visibility for PPL points of a lane (class uniform) */
+template <int PPL>
+__device__ __forceinline__ void vw_row_vis(const ReconArgs& a, int cls, int64_t h, const double* pl,
+                                           const double* pm, const double* pn, const bool* ok, bool* vis) {
+  if (cls == VW_CLS_VISIBLE) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
+    return;
+  }
+  if (cls == VW_CLS_HIDDEN) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) vis[k] = false;
+    return;
+  }
+  const RowRec rec = a.rows[h];
+  const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+  const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+  const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
+  const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k)
+    vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+}
+
+template <int PPL, bool MIXED, int KIND>
+__global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_id = (int64_t)blockIdx.x * 4 + vw_rfl((int)(threadIdx.x >> 6));
+  const int64_t pbase = wave_id * 64 * PPL;
+  if (pbase >= a.npts) return;
+  const int64_t tile = pbase / a.TP;
+  const vw_segparams sp = a.info->sp;
+  const int64_t nu = a.info->nuniq;
+  const int64_t q0 = a.info->first_nz;
+
+  double pl[PPL], pm[PPL], pn[PPL], lp[PPL], mp[PPL];
+  bool ok[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    int64_t p = pbase + lane + 64 * k;
+    ok[k] = p < a.npts;
+    int64_t pp = ok[k] ? p : 0;
+    pl[k] = a.l[pp];
+    pm[k] = a.m[pp];
+    pn[k] = a.n[pp];
+    lp[k] = fma(sp.fa, pn[k], pl[k]);
+    mp[k] = fma(sp.fb, pn[k], pm[k]);
+  }
+  double dacc[PPL];
+  float fs[PPL], fc[PPL], fb[PPL];
+  uint32_t wl[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) { dacc[k] = 0.0; fs[k] = 0.f; fc[k] = 0.f; fb[k] = 0.f; wl[k] = 0; }
+  uint32_t wu = 0;
+  int nblk = 0;
+
+  for (int64_t q = q0; q < nu; ++q) {
+    const GroupRec g = a.grec[q];
+    double vr = g.vr, vi = g.vi;
+    bool vis[PPL];
+    if (KIND == VW_KIND_AVG) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
+    } else if (KIND == VW_KIND_REP) {
+      const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, g.rep);
+      if (cls == VW_CLS_HIDDEN) continue;
+      vr = a.vh[g.rep].x;
+      vi = a.vh[g.rep].y;
+      vw_row_vis<PPL>(a, cls, g.rep, pl, pm, pn, ok, vis);
+      if (cls == VW_CLS_VISIBLE) wu += 1;
+      else {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) wl[k] += vis[k] ? 1u : 0u;
+      }
+    } else {
+      /* generic representative sets (viss_recon_kernel.cuh:603-614) */
+      const int32_t s0 = g.s0, s1 = g.s1, L = s1 - s0;
+      int32_t posv[4];
+      int np;
+      if (a.occ_mode == 0) np = -1;
+      else if (a.occ_mode == 1) { posv[0] = s0; np = 1; }
+      else if (a.occ_mode == 2) { posv[0] = s0 + (L >> 1); np = 1; }
+      else { posv[0] = s0; posv[1] = s0 + L / 3; posv[2] = s0 + (2 * L) / 3; posv[3] = s1 - 1; np = 4; }
+      double sr[PPL], si[PPL];
+      int cn[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) { sr[k] = 0.0; si[k] = 0.0; cn[k] = 0; }
+      const int32_t kend = (np < 0) ? s1 : np;
+      for (int32_t kk = (np < 0 ? s0 : 0); kk < kend; ++kk) {
+        const int32_t pos = (np < 0) ? kk : posv[kk];
+        bool ng;
+        const int64_t h = vw_row_half(a.idx_sorted[pos], a.P, a.full != 0, &ng);
+        const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, h);
+        if (cls == VW_CLS_HIDDEN) continue;
+        bool mv[PPL];
+        vw_row_vis<PPL>(a, cls, h, pl, pm, pn, ok, mv);
+        const double2 v = a.vh[h];
+        const double wr = v.x, wi = ng ? -v.y : v.y;
+        const bool fin = isfinite(wr) && isfinite(wi);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+          if (mv[k] && fin) { sr[k] = sr[k] + wr; si[k] = si[k] + wi; cn[k] += 1; }
+      }
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) any |= cn[k] > 0;
+      if (!__any(any)) continue;
+      double ugu = g.ugu, vgu = g.vgu;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (cn[k] > 0) {
+          const double invc = 1.0 / (double)cn[k];
+          const double x = vw_recon_phase(ugu, vgu, lp[k], mp[k]);
+          const double ar = sr[k] * invc, ai = si[k] * invc;
+          if (MIXED) {
+            const float f = (float)(x - rint(x));
+            float s, c;
+            vw_hw_sincos_turn(f, &s, &c);
+            fb[k] += fmaf((float)ar, c, -((float)ai * s));
+          } else {
+            dacc[k] = dacc[k] + vw_recon_term(ar, ai, x);
+          }
+          wl[k] += 1;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      if (KIND != VW_KIND_GENERIC && vis[k]) {
+        const double x = vw_recon_phase(g.ugu, g.vgu, lp[k], mp[k]);
+        if (MIXED) {
+          const float f = (float)(x - rint(x));
+          float s, c;
+          vw_hw_sincos_turn(f, &s, &c);
+          fb[k] += fmaf((float)vr, c, -((float)vi * s));
+        } else {
+          dacc[k] = dacc[k] + vw_recon_term(vr, vi, x);
+        }
+      }
+    }
+    if (MIXED) {
+      /* compensated fold of the FP32 block every 32 groups */
+      if (++nblk == 32) {
+        nblk = 0;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+          float y = fb[k] - fc[k];
+          float t = fs[k] + y;
+          fc[k] = (t - fs[k]) - y;
+          fs[k] = t;
+          fb[k] = 0.f;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    if (!ok[k]) continue;
+    const int64_t p = pbase + lane + 64 * k;
+    double seg = MIXED ? (((double)fs[k] - (double)fc[k]) + (double)fb[k]) : dacc[k];
+    a.acc[p] = a.acc[p] + seg;
+    if (KIND != VW_KIND_AVG) a.wacc[p] += wu + wl[k];
+  }
+}
+
+/* final maps: C = acc / W (blockage 1) or acc * (1/W_total) (blockage 0) */
+__global__ void k_normalize(const double* __restrict__ acc, const uint32_t* __restrict__ w, int64_t n, int blockage,
+                            const DevInfo* __restrict__ info, const int64_t* __restrict__ scatter,
+                            double* __restrict__ out, uint32_t* __restrict__ wout) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v;
+  if (blockage) v = w[k] > 0 ? acc[k] / (double)w[k] : 0.0;
+  else {
+    int64_t wt = info->wtotal;
+    double inv = wt > 0 ? 1.0 / (double)wt : 0.0;
+    v = acc[k] * inv;
+  }
+  int64_t dst = scatter ? scatter[k] : k;
+  out[dst] = v;
+  wout[dst] = w[k];
+}
+
+#endif
