@@ -199,14 +199,21 @@ VW_HD double vw_cosphi_from_altitude(double R, double h) {
 }
 
 /* One antenna against a tile cone: bit0 = all visible, bit1 = all hidden.
- * classify_tile_cone (vissfast .cuh:91-98). */
+ * classify_tile_cone (vissfast .cuh:91-98): with d = cos(delta) the angle to
+ * the cone axis and A the cone half-angle, the extremes of s.u over the cone
+ * are cos(delta + A) and cos(delta - A) while delta + A <= pi and A <= delta.
+ * Outside that range the cone contains -u (minimum -1) or u (maximum +1);
+ * the reference formula is not conservative there (cones wider than a few
+ * tens of degrees, e.g. 256-pixel tiles at Nside <= 8), so those two cases
+ * clamp to -1 / +1.  For the small tiles of the production configurations
+ * this is the reference's classification unchanged. */
 VW_HD int vw_cone_bits(double ux, double uy, double uz, double cx, double cy, double cz,
                        double cosA, double sinA, double cosphi) {
   double d = vw_clamp1(vw_dot3(ux, uy, uz, cx, cy, cz));
   double sind = sqrt(fmax(0.0, fma(-d, d, 1.0)));
   double ss = sind * sinA;
-  double lower = fma(d, cosA, -ss);
-  double upper = fma(d, cosA, ss);
+  double lower = (cosA + d <= 0.0) ? -1.0 : fma(d, cosA, -ss);
+  double upper = (cosA <= d) ? 1.0 : fma(d, cosA, ss);
   return (lower >= cosphi ? 1 : 0) | (upper < cosphi ? 2 : 0);
 }
 
@@ -221,6 +228,10 @@ VW_HD int vw_pair_class(int bits_a, int bits_b) {
  * minimum of c.s over the tile (computed by the caller after the centre). */
 VW_HD void vw_cone_centre(double sx, double sy, double sz, double* cx, double* cy, double* cz) {
   double nrm = sqrt(vw_dot3(sx, sy, sz, sx, sy, sz));
+  if (!(nrm > 0.0)) { /* directions cancel: any unit axis gives a valid cone */
+    *cx = 0.0; *cy = 0.0; *cz = 1.0;
+    return;
+  }
   *cx = sx / nrm;
   *cy = sy / nrm;
   *cz = sz / nrm;
