@@ -140,3 +140,13 @@ def test_pipeline_mixed_c1():
     assert rel_err(Ig, Io) <= MIXED_TOL, rel_err(Ig, Io)
     print("mixed C1 errors: viss max", max(errs["viss"]), "C", rel_err(Cg, Co), rms_rel_err(Cg, Co),
           "img", rel_err(Ig, Io), rms_rel_err(Ig, Io))
+
+
+@pytest.mark.parametrize("recon_mode", [0, 1])
+def test_large_segment_groups(recon_mode):
+    # 16 antennas x 40 epochs = 4,800 half rows (9,600 full rows) in one segment:
+    # exercises the multi-workgroup stable ranking instead of the LDS sort
+    cfg = config(nside=8, n_ant=16, n_epochs=40, segs=1, img_n=0, recon_mode=recon_mode, freq_hz=3e7)
+    (Cg, Wg, _, _), (Co, Wo, _, _) = run_both(cfg, _sky(12 * 8 * 8, seed=9), _seg_checks(PLAIN_TOL))
+    assert np.array_equal(Wg, Wo)
+    assert rel_err(Cg, Co) <= PLAIN_TOL

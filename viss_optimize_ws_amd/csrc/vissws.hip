@@ -61,14 +61,19 @@ struct Ctx {
   int32_t *d_pa = nullptr, *d_pb = nullptr;
   RowRec* d_rows = nullptr;
   double* d_ant8 = nullptr;
-  uint8_t *d_cls = nullptr, *d_gcls = nullptr;
+  uint8_t *d_cls = nullptr, *d_gcls = nullptr; /* per (tile, half row): sky, image */
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
   int64_t* d_counts = nullptr;
   double2 *d_vraw = nullptr, *d_vh = nullptr, *d_vfull = nullptr;
   int64_t *d_keys = nullptr, *d_keys_sorted = nullptr, *d_gkeys = nullptr;
-  int32_t *d_idx_sorted = nullptr, *d_goff = nullptr;
-  GroupRec* d_grec = nullptr;
+  int32_t *d_idx_sorted = nullptr, *d_goff = nullptr, *d_rank = nullptr;
+  GroupSoA gs{};
+  uint8_t *d_gcls_sky = nullptr, *d_gcls_img = nullptr; /* [ntile][nq_pad] per group */
+  double* d_rpart = nullptr;
+  uint32_t* d_rwpart = nullptr;
+  int64_t nq_pad = 0;
+  bool generic = false; /* recon needs per-row classes of the image tiles */
   DevInfo* d_info = nullptr;
   double *d_Cacc = nullptr, *d_Gacc = nullptr;
   uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
@@ -247,6 +252,8 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   c->ngt = (c->ngp + c->TP - 1) / c->TP;
   c->max_nh = (int64_t)c->max_tlen * c->P;
   c->nh_pad = (c->max_nh + 63) / 64 * 64;
+  c->nq_pad = (2 * c->max_nh + 63) / 64 * 64;
+  c->generic = cfg->blockage && (cfg->recon_mode == VW_RECON_FULL || cfg->occ_mode != 2);
   if (c->max_nh > (int64_t)INT32_MAX / 4) return c->fail(VW_ERR_RANGE, "segment too long for 32-bit row indices");
   return VW_OK;
 }
@@ -271,6 +278,19 @@ VissShape viss_shape(const Ctx* c, int64_t nh) {
   s.bpc = (s.nrb + 3) / 4;
   s.nblocks = s.nchunk * s.bpc;
   return s;
+}
+
+/* recon launch shape: PPL points per lane, group ranges to fill the GPU */
+constexpr int RECON_PPL = 4;
+struct ReconShape {
+  int64_t pwaves, nchunk;
+};
+ReconShape recon_shape(int64_t npts) {
+  ReconShape r;
+  r.pwaves = (npts + 64 * RECON_PPL - 1) / (64 * RECON_PPL);
+  const int64_t target_waves = 8192;
+  r.nchunk = std::max<int64_t>(1, std::min<int64_t>(64, (target_waves + r.pwaves - 1) / r.pwaves));
+  return r;
 }
 
 int upload_day(Ctx* c) {
@@ -379,17 +399,32 @@ int create_device(Ctx* c) {
   if (cls_bytes > 32e9) return c->fail(VW_ERR_RANGE, "tile-class table exceeds 32 GB; use fewer epochs per segment");
   if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
       (rc = dalloc(c, &c->d_cls, (size_t)c->ntile * c->nh_pad)) ||
-      (rc = dalloc(c, &c->d_gcls, (size_t)std::max<int64_t>(c->ngt, 1) * c->nh_pad)) ||
+      (rc = dalloc(c, &c->d_gcls, (size_t)(c->generic ? std::max<int64_t>(c->ngt, 1) : 1) * c->nh_pad)) ||
       (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) || (rc = dalloc(c, &c->d_vraw, mh)) ||
       (rc = dalloc(c, &c->d_vh, mh)) || (rc = dalloc(c, &c->d_vfull, 2 * mh)) ||
       (rc = dalloc(c, &c->d_keys, 2 * mh)) || (rc = dalloc(c, &c->d_keys_sorted, 2 * mh)) ||
       (rc = dalloc(c, &c->d_gkeys, 2 * mh + 1)) || (rc = dalloc(c, &c->d_idx_sorted, 2 * mh)) ||
-      (rc = dalloc(c, &c->d_goff, 2 * mh + 2)) || (rc = dalloc(c, &c->d_grec, 2 * mh + 1)) ||
+      (rc = dalloc(c, &c->d_goff, 2 * mh + 2)) || (rc = dalloc(c, &c->d_rank, 2 * mh)) ||
       (rc = dalloc(c, &c->d_info, 1)) ||
       (rc = dalloc(c, &c->d_out, (size_t)c->npix)) || (rc = dalloc(c, &c->d_wout, (size_t)c->npix)) ||
       (rc = dalloc(c, &c->d_iout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))) ||
       (rc = dalloc(c, &c->d_iwout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))))
     return rc;
+  {
+    const size_t nq = (size_t)c->nq_pad;
+    GroupSoA& g = c->gs;
+    if ((rc = dalloc(c, &g.rec, nq)) || (rc = dalloc(c, &g.ant, 10 * nq)) ||
+        (rc = dalloc(c, &g.gia, nq)) || (rc = dalloc(c, &g.gib, nq)) || (rc = dalloc(c, &g.grep, nq)) ||
+        (rc = dalloc(c, &c->d_gcls_sky, (size_t)c->ntile * nq)) ||
+        (rc = dalloc(c, &c->d_gcls_img, (size_t)std::max<int64_t>(c->ngt, 1) * nq)))
+      return rc;
+    size_t rp = 1;
+    for (int img = 0; img < 2; ++img) {
+      int64_t npts = img ? c->ngp : c->npix;
+      if (npts) rp = std::max(rp, (size_t)(recon_shape(npts).nchunk * npts));
+    }
+    if ((rc = dalloc(c, &c->d_rpart, rp)) || (rc = dalloc(c, &c->d_rwpart, rp))) return rc;
+  }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
   HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * np, c->st));
@@ -420,7 +455,7 @@ int enq_gen(Ctx* c, int t0, int tlen) {
   k_classify<<<blocks_for(c->ntile * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, c->d_rows, c->d_ant8, c->nh,
                                                                   c->nh_pad, c->d_cls);
   KC();
-  if (c->ngp) {
+  if (c->ngp && c->generic) {
     k_classify<<<blocks_for(c->ngt * c->nh_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->d_rows, c->d_ant8, c->nh,
                                                                   c->nh_pad, c->d_gcls);
     KC();
@@ -494,22 +529,44 @@ int enq_group(Ctx* c) {
   KC();
   k_keys<<<blocks_for(nr), 256, 0, c->st>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
-  k_rank_scatter<<<blocks_for(nr), 256, 0, c->st>>>(c->d_keys, nr, c->d_idx_sorted, c->d_keys_sorted);
-  KC();
+  if (nr <= VW_SORT_SMALL) {
+    k_sort_small<<<1, 1024, 0, c->st>>>(c->d_keys, nr, c->d_idx_sorted, c->d_keys_sorted);
+    KC();
+  } else {
+    HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->st));
+    dim3 grid(blocks_for(nr), (unsigned)((nr + VW_RANK_JBLK - 1) / VW_RANK_JBLK));
+    k_rank_count<<<grid, 256, 0, c->st>>>(c->d_keys, nr, c->d_rank);
+    KC();
+    k_rank_scatter<<<blocks_for(nr), 256, 0, c->st>>>(c->d_keys, c->d_rank, nr, c->d_idx_sorted, c->d_keys_sorted);
+    KC();
+  }
   k_groups<<<1, 1024, 0, c->st>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
   KC();
-  k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->P, full ? 1 : 0,
-                                                  c->d_info, c->d_grec);
+  k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
+                                                  c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->d_info, c->gs);
   KC();
+  if (c->cfg.blockage && !c->generic) {
+    /* per-(tile, group) classes for the representative kernel */
+    if (c->cfg.healpix_out) {
+      k_group_cls_gather<<<blocks_for(c->ntile * c->nq_pad), 256, 0, c->st>>>(c->d_cls, c->ntile, c->nh_pad, c->gs.grep,
+                                                                              c->d_info, c->nq_pad, c->d_gcls_sky);
+      KC();
+    }
+    if (c->ngp) {
+      k_group_cls<<<blocks_for(c->ngt * c->nq_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->gs.gia, c->gs.gib, c->d_ant8,
+                                                                     c->d_info, c->nq_pad, c->d_gcls_img);
+      KC();
+    }
+  }
   ev_end(c, ST_GROUP);
   c->have_groups = true;
   return VW_OK;
 }
 
-template <int PPL, bool MIXED, int KIND>
+template <bool MIXED, int KIND>
 void launch_recon(const ReconArgs& a, hipStream_t st) {
-  int64_t waves = (a.npts + 64 * PPL - 1) / (64 * PPL);
-  k_recon<PPL, MIXED, KIND><<<blocks_for(waves, 4), 256, 0, st>>>(a);
+  int64_t waves = a.pwaves * a.nchunk;
+  k_recon<RECON_PPL, MIXED, KIND><<<blocks_for(waves, 4), 256, 0, st>>>(a);
 }
 
 int enq_recon_set(Ctx* c, bool img) {
@@ -519,36 +576,47 @@ int enq_recon_set(Ctx* c, bool img) {
   a.n = img ? c->d_gn : c->d_n;
   a.npts = img ? c->ngp : c->npix;
   a.TP = c->TP;
-  a.cls = (const uint32_t*)(img ? c->d_gcls : c->d_cls);
+  a.gcls = (const uint32_t*)(img ? c->d_gcls_img : c->d_gcls_sky);
+  a.nq_pad = c->nq_pad;
+  a.rcls = (const uint32_t*)(img ? c->d_gcls : c->d_cls);
   a.nh_pad = c->nh_pad;
-  a.rows = c->d_rows;
-  a.ant8 = c->d_ant8;
-  a.grec = c->d_grec;
+  a.g = c->gs;
+  a.goff = c->d_goff;
   a.idx_sorted = c->d_idx_sorted;
   a.vh = c->d_vh;
+  a.rows = c->d_rows;
+  a.ant8 = c->d_ant8;
   a.info = c->d_info;
   a.P = c->P;
   a.full = c->cfg.recon_mode == VW_RECON_FULL;
   a.occ_mode = c->cfg.occ_mode;
-  a.acc = img ? c->d_Gacc : c->d_Cacc;
-  a.wacc = img ? c->d_GWacc : c->d_Wacc;
+  ReconShape rs = recon_shape(a.npts);
+  a.pwaves = rs.pwaves;
+  a.nchunk = rs.nchunk;
+  a.pstride = a.npts;
+  a.part = c->d_rpart;
+  a.wpart = c->d_rwpart;
   int kind;
   if (!c->cfg.blockage) kind = VW_KIND_AVG;
-  else if (!a.full && c->cfg.occ_mode == 2) kind = VW_KIND_REP;
+  else if (!c->generic) kind = VW_KIND_REP;
   else kind = VW_KIND_GENERIC;
   bool mixed = c->cfg.precision == VW_PREC_MIXED;
   ev_begin(c, ST_RECONK);
   if (mixed) {
-    if (kind == VW_KIND_REP) launch_recon<1, true, VW_KIND_REP>(a, c->st);
-    else if (kind == VW_KIND_GENERIC) launch_recon<1, true, VW_KIND_GENERIC>(a, c->st);
-    else launch_recon<1, true, VW_KIND_AVG>(a, c->st);
+    if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, c->st);
+    else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, c->st);
+    else launch_recon<true, VW_KIND_AVG>(a, c->st);
   } else {
-    if (kind == VW_KIND_REP) launch_recon<1, false, VW_KIND_REP>(a, c->st);
-    else if (kind == VW_KIND_GENERIC) launch_recon<1, false, VW_KIND_GENERIC>(a, c->st);
-    else launch_recon<1, false, VW_KIND_AVG>(a, c->st);
+    if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, c->st);
+    else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, c->st);
+    else launch_recon<false, VW_KIND_AVG>(a, c->st);
   }
   KC();
   ev_end(c, ST_RECONK);
+  k_recon_reduce<<<blocks_for(a.npts), 256, 0, c->st>>>(a.part, a.wpart, a.nchunk, a.pstride, a.npts,
+                                                        c->cfg.blockage, img ? c->d_Gacc : c->d_Cacc,
+                                                        img ? c->d_GWacc : c->d_Wacc);
+  KC();
   return VW_OK;
 }
 
@@ -910,7 +978,20 @@ VW_API int vw_get_tile_classes(void* p, int32_t img, uint8_t* cls) {
   if (img && !c->ngp) return c->fail(VW_ERR_ARG, "no lm grid");
   int64_t nt = img ? c->ngt : c->ntile;
   std::vector<uint8_t> t((size_t)(nt * c->nh_pad));
-  HC(hipMemcpy(t.data(), img ? c->d_gcls : c->d_cls, t.size(), hipMemcpyDeviceToHost));
+  if (img && !c->generic) {
+    /* the image row classes are only kept for the generic recon; build them */
+    uint8_t* d = nullptr;
+    HC(hipMalloc(&d, t.size()));
+    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->d_rows, c->d_ant8, c->nh,
+                                                              c->nh_pad, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(t.data(), d, t.size(), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("vw_get_tile_classes: ") + hipGetErrorString(e));
+  } else {
+    HC(hipMemcpy(t.data(), img ? c->d_gcls : c->d_cls, t.size(), hipMemcpyDeviceToHost));
+  }
   for (int64_t h = 0; h < c->nh; ++h)
     for (int64_t k = 0; k < nt; ++k) cls[h * nt + k] = t[k * c->nh_pad + h];
   return VW_OK;

@@ -29,13 +29,6 @@ struct RowRec {
   int32_t ia, ib; /* antenna-frame indices g*A + a of the two endpoints */
 };
 
-struct GroupRec {
-  double ugu, vgu, vr, vi; /* cell centre (wavelengths) and visibility   */
-  int32_t rep;             /* representative half row (mirror occ 2)     */
-  int32_t s0, s1;          /* member range in idx_sorted                 */
-  int32_t key_nz;          /* key != 0                                   */
-};
-
 struct DevInfo {
   vw_segparams sp;
   int64_t nuniq;
@@ -310,8 +303,28 @@ __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
   *c = __builtin_amdgcn_cosf(r);
 }
 
+/* one term: accumulate bb * exp(-j 2 pi x) (sign of the imaginary part is
+ * applied when the partial is written) */
+template <bool MIXED>
+__device__ __forceinline__ void vw_viss_term(const RowRec& rec, double l, double m, double nm1, double bd, float bf,
+                                             float& fre, float& fim, double& dre, double& dim) {
+  const double x = vw_viss_phase(rec.u, rec.v, rec.w, l, m, nm1);
+  if (MIXED) {
+    const float f = (float)(x - rint(x));
+    float s, c;
+    vw_hw_sincos_turn(f, &s, &c);
+    fre = fmaf(bf, c, fre);
+    fim = fmaf(bf, s, fim);
+  } else {
+    double s, c;
+    vw_sincos_turn(x, &s, &c);
+    dre = fma(bd, c, dre);
+    dim = fma(bd, s, dim);
+  }
+}
+
 template <int TP, int RPW, bool MIXED, bool COUNT>
-__global__ __launch_bounds__(256) void k_viss(VissArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_viss(VissArgs a) {
   constexpr int PPL = TP / 64;
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t chunk = vb / a.bpc;
@@ -341,6 +354,7 @@ __global__ __launch_bounds__(256) void k_viss(VissArgs a) {
       pm[k] = a.m[pp];
       pn[k] = a.n[pp];
       pz[k] = a.nm1[pp];
+      /* weights are zero outside the sky, so the visible path needs no mask */
       if (MIXED) pbf[k] = ok[k] ? a.Bf[pp] : 0.0f;
       else pb[k] = ok[k] ? a.B[pp] : 0.0;
     }
@@ -349,40 +363,27 @@ __global__ __launch_bounds__(256) void k_viss(VissArgs a) {
       const int64_t h = r0 + r;
       if (h < a.nh) {
         const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, h);
-        if (cls != VW_CLS_HIDDEN) {
+        if (cls == VW_CLS_VISIBLE) {
           const RowRec rec = a.rows[h];
-          bool vis[PPL];
-          if (cls == VW_CLS_MIXED) {
-            const double* qa = a.ant8 + (int64_t)rec.ia * 8;
-            const double* qb = a.ant8 + (int64_t)rec.ib * 8;
-            const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
-            const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-              vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                       vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
-          } else {
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
-          }
 #pragma unroll
           for (int k = 0; k < PPL; ++k) {
-            const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl[k], pm[k], pz[k]);
-            if (MIXED) {
-              const float f = (float)(x - rint(x));
-              float s, c;
-              vw_hw_sincos_turn(f, &s, &c);
-              const float bb = vis[k] ? pbf[k] : 0.0f;
-              fre[r] = fmaf(bb, c, fre[r]);
-              fim[r] = fmaf(bb, s, fim[r]);
-            } else {
-              double s, c;
-              vw_sincos_turn(x, &s, &c);
-              const double bb = vis[k] ? pb[k] : 0.0;
-              dre[r] = fma(bb, c, dre[r]);
-              dim[r] = fma(bb, s, dim[r]);
-            }
-            if (COUNT) cnt[r] += vis[k] ? 1 : 0;
+            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], MIXED ? 0.0 : pb[k], MIXED ? pbf[k] : 0.f, fre[r], fim[r],
+                                dre[r], dim[r]);
+            if (COUNT) cnt[r] += ok[k] ? 1 : 0;
+          }
+        } else if (cls == VW_CLS_MIXED) {
+          const RowRec rec = a.rows[h];
+          const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+          const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+          const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
+          const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) {
+            const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], vis ? pb[k] : 0.0, vis ? pbf[k] : 0.f, fre[r], fim[r],
+                                dre[r], dim[r]);
+            if (COUNT) cnt[r] += vis ? 1 : 0;
           }
         }
       }
@@ -490,29 +491,77 @@ __global__ void k_keys(const RowRec* __restrict__ rows, int64_t nr, int P, int f
   keys[r] = vw_grid_key(sg * rec.u, sg * rec.v, &sp, full ? 0 : 1);
 }
 
-/* stable rank by (key, index); all-pairs count with LDS-staged key tiles */
-__global__ __launch_bounds__(256) void k_rank_scatter(const int64_t* __restrict__ keys, int64_t nr,
-                                                      int32_t* __restrict__ idx_sorted,
-                                                      int64_t* __restrict__ keys_sorted) {
+/* Stable order by (key, row index).  Up to SORT_SMALL rows: one workgroup,
+ * bitonic network over (key, index) pairs held in LDS (padding sorts last).
+ * Larger: all-pairs rank counted over a 2-D grid of (row block, key block)
+ * with integer atomics (exact, order-independent), then a scatter. */
+#define VW_SORT_SMALL 4096
+__global__ __launch_bounds__(1024) void k_sort_small(const int64_t* __restrict__ keys, int64_t nr,
+                                                     int32_t* __restrict__ idx_sorted,
+                                                     int64_t* __restrict__ keys_sorted) {
+  __shared__ int64_t sk[VW_SORT_SMALL];
+  __shared__ int32_t si[VW_SORT_SMALL];
+  const int t = threadIdx.x;
+  int npad = 2;
+  while (npad < nr) npad <<= 1;
+  for (int i = t; i < npad; i += 1024) {
+    sk[i] = i < nr ? keys[i] : INT64_MAX;
+    si[i] = i;
+  }
+  __syncthreads();
+  for (int k = 2; k <= npad; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < npad; i += 1024) {
+        int x = i ^ j;
+        if (x > i) {
+          int64_t ka = sk[i], kb = sk[x];
+          int32_t ia = si[i], ib = si[x];
+          bool gt = ka > kb || (ka == kb && ia > ib);
+          bool up = (i & k) == 0;
+          if (gt == up) {
+            sk[i] = kb; sk[x] = ka;
+            si[i] = ib; si[x] = ia;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = t; i < nr; i += 1024) {
+    idx_sorted[i] = si[i];
+    keys_sorted[i] = sk[i];
+  }
+}
+
+#define VW_RANK_JBLK 2048
+__global__ __launch_bounds__(256) void k_rank_count(const int64_t* __restrict__ keys, int64_t nr,
+                                                    int32_t* __restrict__ rank) {
   __shared__ int64_t sk[256];
-  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  int64_t ki = i < nr ? keys[i] : 0;
-  int64_t rank = 0;
-  for (int64_t base = 0; base < nr; base += 256) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * VW_RANK_JBLK;
+  const int64_t j1 = j0 + VW_RANK_JBLK < nr ? j0 + VW_RANK_JBLK : nr;
+  const int64_t ki = i < nr ? keys[i] : 0;
+  int32_t cnt = 0;
+  for (int64_t base = j0; base < j1; base += 256) {
     int64_t j = base + threadIdx.x;
-    sk[threadIdx.x] = j < nr ? keys[j] : INT64_MAX;
+    sk[threadIdx.x] = j < j1 ? keys[j] : INT64_MAX;
     __syncthreads();
-    int64_t lim = nr - base < 256 ? nr - base : 256;
+    int64_t lim = j1 - base < 256 ? j1 - base : 256;
     for (int64_t q = 0; q < lim; ++q) {
       int64_t kj = sk[q];
-      rank += (kj < ki) || (kj == ki && base + q < i);
+      cnt += (kj < ki) || (kj == ki && base + q < i);
     }
     __syncthreads();
   }
-  if (i < nr) {
-    idx_sorted[rank] = (int32_t)i;
-    keys_sorted[rank] = ki;
-  }
+  if (i < nr && cnt) atomicAdd(&rank[i], cnt);
+}
+
+__global__ void k_rank_scatter(const int64_t* __restrict__ keys, const int32_t* __restrict__ rank, int64_t nr,
+                               int32_t* __restrict__ idx_sorted, int64_t* __restrict__ keys_sorted) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  idx_sorted[rank[i]] = (int32_t)i;
+  keys_sorted[rank[i]] = keys[i];
 }
 
 /* run boundaries -> keys, offsets, nuniq (single workgroup) */
@@ -550,36 +599,93 @@ __global__ __launch_bounds__(1024) void k_groups(const int64_t* __restrict__ ks,
   }
 }
 
+/* Per-group records as SoA (uniform loads of four groups per array in one
+ * scalar load): cell centre, visibility (representative or group mean), the
+ * representative's antenna frames and member range. */
+struct GRec {
+  double ugu, vgu, vr, vi;
+};
+struct GroupSoA {
+  GRec* rec;    /* cell centre (wavelengths) and visibility, 32 B      */
+  double* ant;  /* [q][10]: representative's antenna frames a and b    */
+  int32_t *gia, *gib, *grep;
+};
+
 __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* __restrict__ goff,
-                             const int32_t* __restrict__ idx_sorted, const double2* __restrict__ vh, int P, int full,
-                             const DevInfo* __restrict__ info, GroupRec* __restrict__ grec) {
+                             const int32_t* __restrict__ idx_sorted, const double2* __restrict__ vh,
+                             const RowRec* __restrict__ rows, const double* __restrict__ ant8, int P, int full,
+                             int blockage,
+                             const DevInfo* __restrict__ info, GroupSoA g) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= info->nuniq) return;
   vw_segparams sp = info->sp;
-  GroupRec g;
   int64_t key = gkeys[q];
   int32_t s0 = goff[q], s1 = goff[q + 1];
-  g.s0 = s0;
-  g.s1 = s1;
-  g.key_nz = key != 0;
-  g.ugu = 0.0;
-  g.vgu = 0.0;
-  if (key != 0) vw_key_uv(key, &sp, &g.ugu, &g.vgu);
+  double ugu = 0.0, vgu = 0.0;
+  if (key != 0) vw_key_uv(key, &sp, &ugu, &vgu);
   bool neg;
   int64_t hr = vw_row_half(idx_sorted[s0 + (s1 - s0) / 2], P, full != 0, &neg);
-  g.rep = (int32_t)hr;
-  /* group mean (blockage 0) in member order; representative value otherwise */
-  double sr = 0.0, si = 0.0;
-  for (int32_t k = s0; k < s1; ++k) {
-    bool ng;
-    int64_t h = vw_row_half(idx_sorted[k], P, full != 0, &ng);
-    double2 v = vh[h];
-    sr = sr + v.x;
-    si = si + (ng ? -v.y : v.y);
+  double vr, vi;
+  if (blockage) {
+    /* representative value (mirror occ 2 uses it directly) */
+    double2 v = vh[hr];
+    vr = v.x;
+    vi = neg ? -v.y : v.y;
+  } else {
+    /* group mean over members in sorted order (compute_avg, viss_recon_kernel.cuh:327-333) */
+    double sr = 0.0, si = 0.0;
+    for (int32_t k = s0; k < s1; ++k) {
+      bool ng;
+      int64_t h = vw_row_half(idx_sorted[k], P, full != 0, &ng);
+      double2 v = vh[h];
+      sr = sr + v.x;
+      si = si + (ng ? -v.y : v.y);
+    }
+    vr = sr / (double)(s1 - s0);
+    vi = si / (double)(s1 - s0);
   }
-  g.vr = sr / (double)(s1 - s0);
-  g.vi = si / (double)(s1 - s0);
-  grec[q] = g;
+  GRec gr;
+  gr.ugu = ugu;
+  gr.vgu = vgu;
+  gr.vr = vr;
+  gr.vi = vi;
+  g.rec[q] = gr;
+  g.grep[q] = (int32_t)hr;
+  RowRec r = rows[hr];
+  g.gia[q] = r.ia;
+  g.gib[q] = r.ib;
+  const double* qa = ant8 + (int64_t)r.ia * 8;
+  const double* qb = ant8 + (int64_t)r.ib * 8;
+  double* d = g.ant + q * 10;
+  d[0] = qa[0]; d[1] = qa[1]; d[2] = qa[2]; d[3] = qa[3]; d[4] = qa[7];
+  d[5] = qb[0]; d[6] = qb[1]; d[7] = qb[2]; d[8] = qb[3]; d[9] = qb[7];
+}
+
+/* class of (output tile, group) = class of (tile, representative row) */
+__global__ void k_group_cls_gather(const uint8_t* __restrict__ rcls, int64_t ntile, int64_t nh_pad,
+                                   const int32_t* __restrict__ grep, const DevInfo* __restrict__ info,
+                                   int64_t nq_pad, uint8_t* __restrict__ gcls) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ntile * nq_pad) return;
+  int64_t t = k / nq_pad, q = k - t * nq_pad;
+  gcls[k] = q < info->nuniq ? rcls[t * nh_pad + grep[q]] : (uint8_t)VW_CLS_HIDDEN;
+}
+
+/* class of (output tile, group) from the representative's antennas */
+__global__ void k_group_cls(const double* __restrict__ cone, int64_t ntile, const int32_t* __restrict__ gia,
+                            const int32_t* __restrict__ gib, const double* __restrict__ ant8,
+                            const DevInfo* __restrict__ info, int64_t nq_pad, uint8_t* __restrict__ gcls) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ntile * nq_pad) return;
+  int64_t t = k / nq_pad, q = k - t * nq_pad;
+  if (q >= info->nuniq) { gcls[k] = VW_CLS_HIDDEN; return; }
+  const double* qa = ant8 + (int64_t)gia[q] * 8;
+  const double* qb = ant8 + (int64_t)gib[q] * 8;
+  double cx = cone[t], cy = cone[ntile + t], cz = cone[2 * ntile + t];
+  double ca = cone[3 * ntile + t], sa = cone[4 * ntile + t];
+  int ba = vw_cone_bits(qa[4], qa[5], qa[6], cx, cy, cz, ca, sa, qa[7]);
+  int bb = vw_cone_bits(qb[4], qb[5], qb[6], cx, cy, cz, ca, sa, qb[7]);
+  gcls[k] = (uint8_t)vw_pair_class(ba, bb);
 }
 
 /* ------------------------------------------------------------------ */
@@ -588,40 +694,35 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
 struct ReconArgs {
   const double *l, *m, *n;
   int64_t npts, TP;
-  const uint32_t* cls; /* [ntile_out][nh_pad] */
+  const uint32_t* gcls; /* [ntile_out][nq_pad]: per group (representative kind) */
+  int64_t nq_pad;
+  const uint32_t* rcls; /* [ntile_out][nh_pad]: per half row (generic kind)     */
   int64_t nh_pad;
-  const RowRec* rows;
-  const double* ant8;
-  const GroupRec* grec;
+  GroupSoA g;
+  const int32_t* goff;
   const int32_t* idx_sorted;
   const double2* vh;
+  const RowRec* rows;
+  const double* ant8;
   const DevInfo* info;
   int P, full, occ_mode;
-  double* acc;   /* accumulated map (npts)        */
-  uint32_t* wacc; /* accumulated weights (npts)     */
+  int64_t nchunk;     /* group ranges per point block                     */
+  int64_t pwaves;     /* point blocks (waves) per group range             */
+  int64_t pstride;    /* partial row stride                               */
+  double* part;       /* [nchunk][pstride]                                */
+  uint32_t* wpart;    /* [nchunk][pstride]                                */
 };
 
 #define VW_KIND_REP 0
 #define VW_KIND_GENERIC 1
 #define VW_KIND_AVG 2
 
-/* one row's effective visibility for PPL points of a lane (class uniform) */
+/* visibility of one row for the PPL points of a lane given the class */
 template <int PPL>
-__device__ __forceinline__ void vw_row_vis(const ReconArgs& a, int cls, int64_t h, const double* pl,
+__device__ __forceinline__ void vw_ant_vis(const double* __restrict__ ant8, int ia, int ib, const double* pl,
                                            const double* pm, const double* pn, const bool* ok, bool* vis) {
-  if (cls == VW_CLS_VISIBLE) {
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
-    return;
-  }
-  if (cls == VW_CLS_HIDDEN) {
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) vis[k] = false;
-    return;
-  }
-  const RowRec rec = a.rows[h];
-  const double* qa = a.ant8 + (int64_t)rec.ia * 8;
-  const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+  const double* qa = ant8 + (int64_t)ia * 8;
+  const double* qb = ant8 + (int64_t)ib * 8;
   const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
   const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
 #pragma unroll
@@ -630,16 +731,58 @@ __device__ __forceinline__ void vw_row_vis(const ReconArgs& a, int cls, int64_t 
              vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
 }
 
+template <bool MIXED>
+struct ReconAcc;
+
+template <>
+struct ReconAcc<false> {
+  double a = 0.0;
+  __device__ void term(double ugu, double vgu, double vr, double vi, float, float, double lp, double mp) {
+    a = a + vw_recon_term(vr, vi, vw_recon_phase(ugu, vgu, lp, mp));
+  }
+  __device__ void fold() {}
+  __device__ double value() const { return a; }
+};
+
+/* FP64 phase and turn reduction, hardware FP32 sin/cos, FP32 block sums
+ * folded with Kahan compensation */
+template <>
+struct ReconAcc<true> {
+  float s = 0.f, c = 0.f, b = 0.f;
+  __device__ void term(double ugu, double vgu, double, double, float vrf, float vif, double lp, double mp) {
+    const double x = vw_recon_phase(ugu, vgu, lp, mp);
+    const float f = (float)(x - rint(x));
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+    b = fmaf(vrf, cs, fmaf(-vif, sn, b));
+  }
+  __device__ void fold() {
+    float y = b - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+    b = 0.f;
+  }
+  __device__ double value() const { return ((double)s - (double)c) + (double)b; }
+};
+
 template <int PPL, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave_id = (int64_t)blockIdx.x * 4 + vw_rfl((int)(threadIdx.x >> 6));
-  const int64_t pbase = wave_id * 64 * PPL;
-  if (pbase >= a.npts) return;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + vw_rfl((int)(threadIdx.x >> 6));
+  const int64_t chunk = wid / a.pwaves;
+  const int64_t pw = wid - chunk * a.pwaves;
+  if (chunk >= a.nchunk) return;
+  const int64_t pbase = pw * 64 * PPL;
   const int64_t tile = pbase / a.TP;
   const vw_segparams sp = a.info->sp;
   const int64_t nu = a.info->nuniq;
   const int64_t q0 = a.info->first_nz;
+  /* group range of this chunk, aligned to 4 groups */
+  const int64_t base = q0 & ~(int64_t)3;
+  const int64_t span = ((nu - base) + a.nchunk - 1) / a.nchunk;
+  const int64_t spanA = (span + 3) & ~(int64_t)3;
+  const int64_t qa_ = base + chunk * spanA;
+  const int64_t qb_ = qa_ + spanA < nu ? qa_ + spanA : nu;
 
   double pl[PPL], pm[PPL], pn[PPL], lp[PPL], mp[PPL];
   bool ok[PPL];
@@ -654,35 +797,69 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
     lp[k] = fma(sp.fa, pn[k], pl[k]);
     mp[k] = fma(sp.fb, pn[k], pm[k]);
   }
-  double dacc[PPL];
-  float fs[PPL], fc[PPL], fb[PPL];
+  ReconAcc<MIXED> acc[PPL];
   uint32_t wl[PPL];
 #pragma unroll
-  for (int k = 0; k < PPL; ++k) { dacc[k] = 0.0; fs[k] = 0.f; fc[k] = 0.f; fb[k] = 0.f; wl[k] = 0; }
+  for (int k = 0; k < PPL; ++k) wl[k] = 0;
   uint32_t wu = 0;
   int nblk = 0;
 
-  for (int64_t q = q0; q < nu; ++q) {
-    const GroupRec g = a.grec[q];
-    double vr = g.vr, vi = g.vi;
-    bool vis[PPL];
-    if (KIND == VW_KIND_AVG) {
+  if (KIND == VW_KIND_REP || KIND == VW_KIND_AVG) {
+    /* blocks of 4 groups; the next block's records and class word are
+     * fetched before the current block is evaluated */
+    GRec nx[4];
+    uint32_t ncw = 0x01010101u; /* AVG: every group counts, no class */
+    if (qa_ < qb_) {
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) vis[k] = ok[k];
-    } else if (KIND == VW_KIND_REP) {
-      const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, g.rep);
-      if (cls == VW_CLS_HIDDEN) continue;
-      vr = a.vh[g.rep].x;
-      vi = a.vh[g.rep].y;
-      vw_row_vis<PPL>(a, cls, g.rep, pl, pm, pn, ok, vis);
-      if (cls == VW_CLS_VISIBLE) wu += 1;
-      else {
+      for (int j = 0; j < 4; ++j) nx[j] = a.g.rec[qa_ + j];
+      if (KIND == VW_KIND_REP) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + qa_) >> 2]);
+    }
+    for (int64_t q4 = qa_; q4 < qb_; q4 += 4) {
+      GRec cur[4];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) wl[k] += vis[k] ? 1u : 0u;
+      for (int j = 0; j < 4; ++j) cur[j] = nx[j];
+      const uint32_t cw = ncw;
+      if (q4 + 4 < qb_) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nx[j] = a.g.rec[q4 + 4 + j];
+        if (KIND == VW_KIND_REP) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + q4 + 4) >> 2]);
       }
-    } else {
-      /* generic representative sets (viss_recon_kernel.cuh:603-614) */
-      const int32_t s0 = g.s0, s1 = g.s1, L = s1 - s0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t q = q4 + j;
+        const int cls = (int)((cw >> (8 * j)) & 0xffu);
+        if (q < q0 || q >= qb_ || cls == VW_CLS_HIDDEN) continue;
+        const GRec g = cur[j];
+        const float vrf = (float)g.vr, vif = (float)g.vi;
+        if (cls == VW_CLS_VISIBLE) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k]);
+          wu += 1;
+        } else {
+          const double* d = a.g.ant + q * 10;
+          const double ax = d[0], ay = d[1], az = d[2], ai = d[3], ac = d[4];
+          const double bx = d[5], by = d[6], bz = d[7], bi = d[8], bc = d[9];
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) {
+            const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+            if (vis) {
+              acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k]);
+              wl[k] += 1;
+            }
+          }
+        }
+        if (MIXED && ++nblk == 32) {
+          nblk = 0;
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) acc[k].fold();
+        }
+      }
+    }
+  } else {
+    /* generic representative sets (viss_recon_kernel.cuh:603-614, isfinite filter) */
+    for (int64_t q = (qa_ > q0 ? qa_ : q0); q < qb_; ++q) {
+      const int32_t s0 = a.goff[q], s1 = a.goff[q + 1], L = s1 - s0;
       int32_t posv[4];
       int np;
       if (a.occ_mode == 0) np = -1;
@@ -698,10 +875,16 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
         const int32_t pos = (np < 0) ? kk : posv[kk];
         bool ng;
         const int64_t h = vw_row_half(a.idx_sorted[pos], a.P, a.full != 0, &ng);
-        const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, h);
+        const int cls = vw_cls_lookup(a.rcls, tile, a.nh_pad, h);
         if (cls == VW_CLS_HIDDEN) continue;
         bool mv[PPL];
-        vw_row_vis<PPL>(a, cls, h, pl, pm, pn, ok, mv);
+        if (cls == VW_CLS_VISIBLE) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) mv[k] = ok[k];
+        } else {
+          const RowRec rec = a.rows[h];
+          vw_ant_vis<PPL>(a.ant8, rec.ia, rec.ib, pl, pm, pn, ok, mv);
+        }
         const double2 v = a.vh[h];
         const double wr = v.x, wi = ng ? -v.y : v.y;
         const bool fin = isfinite(wr) && isfinite(wi);
@@ -709,55 +892,20 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
         for (int k = 0; k < PPL; ++k)
           if (mv[k] && fin) { sr[k] = sr[k] + wr; si[k] = si[k] + wi; cn[k] += 1; }
       }
-      bool any = false;
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) any |= cn[k] > 0;
-      if (!__any(any)) continue;
-      double ugu = g.ugu, vgu = g.vgu;
+      const double ugu = a.g.rec[q].ugu, vgu = a.g.rec[q].vgu;
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
         if (cn[k] > 0) {
           const double invc = 1.0 / (double)cn[k];
-          const double x = vw_recon_phase(ugu, vgu, lp[k], mp[k]);
           const double ar = sr[k] * invc, ai = si[k] * invc;
-          if (MIXED) {
-            const float f = (float)(x - rint(x));
-            float s, c;
-            vw_hw_sincos_turn(f, &s, &c);
-            fb[k] += fmaf((float)ar, c, -((float)ai * s));
-          } else {
-            dacc[k] = dacc[k] + vw_recon_term(ar, ai, x);
-          }
+          acc[k].term(ugu, vgu, ar, ai, (float)ar, (float)ai, lp[k], mp[k]);
           wl[k] += 1;
         }
       }
-    }
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-      if (KIND != VW_KIND_GENERIC && vis[k]) {
-        const double x = vw_recon_phase(g.ugu, g.vgu, lp[k], mp[k]);
-        if (MIXED) {
-          const float f = (float)(x - rint(x));
-          float s, c;
-          vw_hw_sincos_turn(f, &s, &c);
-          fb[k] += fmaf((float)vr, c, -((float)vi * s));
-        } else {
-          dacc[k] = dacc[k] + vw_recon_term(vr, vi, x);
-        }
-      }
-    }
-    if (MIXED) {
-      /* compensated fold of the FP32 block every 32 groups */
-      if (++nblk == 32) {
+      if (MIXED && ++nblk == 32) {
         nblk = 0;
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-          float y = fb[k] - fc[k];
-          float t = fs[k] + y;
-          fc[k] = (t - fs[k]) - y;
-          fs[k] = t;
-          fb[k] = 0.f;
-        }
+        for (int k = 0; k < PPL; ++k) acc[k].fold();
       }
     }
   }
@@ -765,10 +913,25 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   for (int k = 0; k < PPL; ++k) {
     if (!ok[k]) continue;
     const int64_t p = pbase + lane + 64 * k;
-    double seg = MIXED ? (((double)fs[k] - (double)fc[k]) + (double)fb[k]) : dacc[k];
-    a.acc[p] = a.acc[p] + seg;
-    if (KIND != VW_KIND_AVG) a.wacc[p] += wu + wl[k];
+    a.part[chunk * a.pstride + p] = acc[k].value();
+    a.wpart[chunk * a.pstride + p] = wu + wl[k];
   }
+}
+
+/* fold group-range partials in order into the accumulators (add_inplace) */
+__global__ void k_recon_reduce(const double* __restrict__ part, const uint32_t* __restrict__ wpart, int64_t nchunk,
+                               int64_t pstride, int64_t npts, int weights, double* __restrict__ acc,
+                               uint32_t* __restrict__ wacc) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  double s = part[p];
+  uint32_t w = wpart[p];
+  for (int64_t k = 1; k < nchunk; ++k) {
+    s = s + part[k * pstride + p];
+    w += wpart[k * pstride + p];
+  }
+  acc[p] = acc[p] + s;
+  if (weights) wacc[p] += w;
 }
 
 /* final maps: C = acc / W (blockage 1) or acc * (1/W_total) (blockage 0) */

@@ -85,14 +85,13 @@ VW_HD void vw_sincos_turn(double r, double* s_out, double* c_out) {
   qq = fma(qq, z, 0x1.5555555555555p-5);   /* +1/4!  */
   qq = fma(qq, z, -0.5);                   /* -1/2!  */
   double cy = fma(z, qq, 1.0);
+  /* quadrant rotation without branches: odd quadrants swap, sign flips on
+   * bit 1 of q (sin) and of q+1 (cos); negation is exact */
   int qi = ((int)q) & 3;
-  double s, c;
-  if (qi == 0) { s = sy; c = cy; }
-  else if (qi == 1) { s = cy; c = -sy; }
-  else if (qi == 2) { s = -sy; c = -cy; }
-  else { s = -cy; c = sy; }
-  *s_out = s;
-  *c_out = c;
+  double a = (qi & 1) ? cy : sy;
+  double b = (qi & 1) ? sy : cy;
+  *s_out = (qi & 2) ? -a : a;
+  *c_out = ((qi + 1) & 2) ? -b : b;
 }
 
 VW_HD double vw_dot3(double ax, double ay, double az, double bx, double by, double bz) {
