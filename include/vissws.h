@@ -70,6 +70,9 @@ typedef struct {
   int32_t healpix_out; /* 1: reconstruct the HEALPix map C                       */
   double r_min_km, r_max_km; /* circular: orbit radius range                     */
   double gm_km3_s2;    /* circular: gravitational parameter of the central body */
+  int64_t class_budget_bytes; /* device bytes for the per-(tile, row) class tables of
+                               * one tile pass (0 = 2 GiB); a smaller budget splits the
+                               * Viss and recon stages into more tile passes      */
 } vw_config;
 
 /* Derived plan (filled by vw_get_plan). */

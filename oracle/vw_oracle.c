@@ -113,6 +113,12 @@ const char* vo_last_error(void* ctx) { return ctx ? ((vo_ctx*)ctx)->err : g_crea
 
 static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
+/* table of (cos, sin)(2 pi k / 256) for the table-driven term sincos */
+static double g_tab[2 * VW_TAB_N];
+static void build_tab(void) {
+  for (int k = 0; k < VW_TAB_N; ++k) vw_sincos_tab_build(k, g_tab);
+}
+
 /* pairwise tile sum in the canonical order: for s = TP/2 .. 1: a[i] += a[i+s] */
 static double tree_sum(double* a, int64_t TP) {
   for (int64_t s = TP / 2; s > 0; s >>= 1)
@@ -224,6 +230,7 @@ int vo_create(const vw_config* cfg, void** out) {
   if (cfg->geometry == VW_GEOM_LUNAR8 && cfg->n_ant != 8) return fail(NULL, VW_ERR_ARG, "lunar geometry has 8 antennas");
   if (cfg->n_ant < 2 || cfg->n_ant > 256) return fail(NULL, VW_ERR_ARG, "n_ant out of range");
   vo_ctx* c = (vo_ctx*)xcalloc(1, sizeof(vo_ctx));
+  build_tab();
   c->cfg = *cfg;
   c->cfg.seg_tlen = NULL;
   c->nside = cfg->nside;
@@ -483,7 +490,7 @@ int vo_viss_half(void* ctx, int64_t row0, int64_t nrows, vw_c128* out, int64_t* 
       for (int64_t p = t * c->TP; p < p1; ++p) {
         if (cls == VW_CLS_MIXED && !predicate(c, h, c->l[p], c->m[p], c->n[p])) continue;
         double s, co;
-        vw_sincos_turn(vw_viss_phase(u, v, w, c->l[p], c->m[p], c->nm1[p]), &s, &co);
+        vw_sincos_turn_tab(vw_viss_phase(u, v, w, c->l[p], c->m[p], c->nm1[p]), g_tab, &s, &co);
         re = re + c->B[p] * co;
         im = im - c->B[p] * s;
         ++cnt;
@@ -626,14 +633,14 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
     vw_key_uv(key, &c->sp, &ugu, &vgu);
     double x = vw_recon_phase(ugu, vgu, lp, mp);
     if (!c->cfg.blockage) {
-      acc = acc + vw_recon_term(c->gavg[q].re, c->gavg[q].im, x);
+      acc = acc + vw_recon_term(c->gavg[q].re, c->gavg[q].im, x, g_tab);
       continue;
     }
     int32_t s0 = c->goff[q], s1 = c->goff[q + 1], L = s1 - s0;
     if (mirror && c->cfg.occ_mode == 2) {
       int64_t h = c->grep[q];
       if (eff_mask(c, h, img, pt)) {
-        acc = acc + vw_recon_term(c->vh[h].re, c->vh[h].im, x);
+        acc = acc + vw_recon_term(c->vh[h].re, c->vh[h].im, x, g_tab);
         w += 1;
       }
       continue;
@@ -657,7 +664,7 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
     }
     if (cnt > 0) {
       double invc = 1.0 / (double)cnt;
-      acc = acc + vw_recon_term(sr * invc, si * invc, x);
+      acc = acc + vw_recon_term(sr * invc, si * invc, x, g_tab);
       w += 1;
     }
   }

@@ -150,3 +150,21 @@ def test_large_segment_groups(recon_mode):
     (Cg, Wg, _, _), (Co, Wo, _, _) = run_both(cfg, _sky(12 * 8 * 8, seed=9), _seg_checks(PLAIN_TOL))
     assert np.array_equal(Wg, Wo)
     assert rel_err(Cg, Co) <= PLAIN_TOL
+
+
+@pytest.mark.parametrize("budget,variant", [(1, dict()), (7000, dict(recon_mode=1, occ_mode=3)),
+                                           (7000, dict(precision=1))])
+def test_tile_passes(budget, variant):
+    # a class budget far below the full per-(tile, row) tables splits the Viss
+    # and recon stages into many tile passes (budget 1: one tile per pass);
+    # results must match the oracle as with a single pass
+    kw = dict(C1)
+    kw.update(variant)
+    cfg = config(class_budget_bytes=budget, **kw)
+    tol = MIXED_TOL if variant.get("precision") else PLAIN_TOL
+    chk = _seg_checks(tol, exact_groups=not variant.get("precision"))
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 32 * 32), chk, segments={0, 2})
+    assert np.array_equal(Wg, Wo)
+    assert np.array_equal(WIg, WIo)
+    assert rel_err(Cg, Co) <= tol
+    assert rel_err(Ig, Io) <= tol

@@ -50,6 +50,7 @@ class vw_config(C.Structure):
         ("r_min_km", C.c_double),
         ("r_max_km", C.c_double),
         ("gm_km3_s2", C.c_double),
+        ("class_budget_bytes", C.c_int64),
     ]
 
 

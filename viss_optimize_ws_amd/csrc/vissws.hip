@@ -61,7 +61,10 @@ struct Ctx {
   int32_t *d_pa = nullptr, *d_pb = nullptr;
   RowRec* d_rows = nullptr;
   double* d_ant8 = nullptr;
-  uint8_t *d_cls = nullptr, *d_gcls = nullptr; /* per (tile, half row): sky, image */
+  uint8_t* d_cls = nullptr; /* class scratch shared by all tile passes (cls_cap bytes) */
+  int64_t cls_cap = 0;       /* bytes                                                    */
+  int64_t vtpp = 0;          /* sky tiles per Viss pass                                  */
+  int64_t rpart_cap = 0;     /* doubles in each recon partial buffer                     */
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
   int64_t* d_counts = nullptr;
@@ -69,7 +72,6 @@ struct Ctx {
   int64_t *d_keys = nullptr, *d_keys_sorted = nullptr, *d_gkeys = nullptr;
   int32_t *d_idx_sorted = nullptr, *d_goff = nullptr, *d_rank = nullptr;
   GroupSoA gs{};
-  uint8_t *d_gcls_sky = nullptr, *d_gcls_img = nullptr; /* [ntile][nq_pad] per group */
   double* d_rpart = nullptr;
   uint32_t* d_rwpart = nullptr;
   int64_t nq_pad = 0;
@@ -258,23 +260,23 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   return VW_OK;
 }
 
-/* Viss launch shape for nh half rows */
+/* Viss launch shape for nh half rows over a pass of nt tiles */
 struct VissShape {
   int rpw;
   int64_t tpc, nchunk, nrb, bpc, nblocks;
 };
-VissShape viss_shape(const Ctx* c, int64_t nh) {
+VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt) {
   VissShape s;
   bool mixed = c->cfg.precision == VW_PREC_MIXED;
   s.rpw = mixed ? 32 : 16;
   s.nrb = (nh + s.rpw - 1) / s.rpw;
   const int64_t target_waves = 8192;
   int64_t nchunk = (target_waves + s.nrb - 1) / s.nrb;
-  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, c->ntile));
-  s.tpc = (c->ntile + nchunk - 1) / nchunk;
+  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, nt));
+  s.tpc = (nt + nchunk - 1) / nchunk;
   const int64_t tpc_cap = mixed ? 16 : 64; /* bounds the FP32 block length per lane */
   if (s.tpc > tpc_cap) s.tpc = tpc_cap;
-  s.nchunk = (c->ntile + s.tpc - 1) / s.tpc;
+  s.nchunk = (nt + s.tpc - 1) / s.tpc;
   s.bpc = (s.nrb + 3) / 4;
   s.nblocks = s.nchunk * s.bpc;
   return s;
@@ -291,6 +293,11 @@ ReconShape recon_shape(int64_t npts) {
   const int64_t target_waves = 8192;
   r.nchunk = std::max<int64_t>(1, std::min<int64_t>(64, (target_waves + r.pwaves - 1) / r.pwaves));
   return r;
+}
+
+/* output tiles per recon pass: as many as the class scratch holds */
+int64_t recon_pass_tiles(const Ctx* c, int64_t ntile_all, int64_t kind_w) {
+  return std::max<int64_t>(1, std::min<int64_t>(ntile_all, c->cls_cap / kind_w));
 }
 
 int upload_day(Ctx* c) {
@@ -390,16 +397,24 @@ int create_device(Ctx* c) {
   HC(hipMemcpyAsync(c->d_pb, c->pb.data(), sizeof(int32_t) * c->P, hipMemcpyHostToDevice, c->st));
   if (!c->circ.empty())
     HC(hipMemcpyAsync(c->d_circ, c->circ.data(), sizeof(double) * c->circ.size(), hipMemcpyHostToDevice, c->st));
-  /* segment buffers sized for the longest segment */
+  /* segment buffers sized for the longest segment; one class scratch buffer
+   * serves every tile pass (rows x tiles for Viss, groups x tiles for recon) */
   const size_t mh = (size_t)c->max_nh;
+  const int64_t kind_w = c->generic ? c->nh_pad : c->nq_pad; /* recon class row width */
+  const int64_t need = std::max<int64_t>(c->ntile * c->nh_pad, std::max<int64_t>(c->ntile, c->ngt) * kind_w);
+  const int64_t budget = c->cfg.class_budget_bytes > 0 ? c->cfg.class_budget_bytes : (int64_t)2 << 30;
+  c->cls_cap = std::min<int64_t>(need, budget);
+  if (c->cls_cap < std::max(c->nh_pad, kind_w)) c->cls_cap = std::max(c->nh_pad, kind_w);
+  c->cls_cap = (c->cls_cap + 3) / 4 * 4;
+  c->vtpp = std::max<int64_t>(1, std::min<int64_t>(c->ntile, c->cls_cap / c->nh_pad));
+  /* partial buffers fit the largest pass shape: a full pass and the last one */
   int64_t nch = 1;
-  for (int k = 0; k < c->segs; ++k) nch = std::max(nch, viss_shape(c, (int64_t)c->seg_tlen[k] * c->P).nchunk);
+  const int64_t vlast = c->ntile - (c->ntile - 1) / c->vtpp * c->vtpp;
+  for (int k = 0; k < c->segs; ++k)
+    for (int64_t nt : {c->vtpp, vlast}) nch = std::max(nch, viss_shape(c, (int64_t)c->seg_tlen[k] * c->P, nt).nchunk);
   c->nchunk_max = nch;
-  const double cls_bytes = (double)(c->ntile + c->ngt) * (double)c->nh_pad;
-  if (cls_bytes > 32e9) return c->fail(VW_ERR_RANGE, "tile-class table exceeds 32 GB; use fewer epochs per segment");
   if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
-      (rc = dalloc(c, &c->d_cls, (size_t)c->ntile * c->nh_pad)) ||
-      (rc = dalloc(c, &c->d_gcls, (size_t)(c->generic ? std::max<int64_t>(c->ngt, 1) : 1) * c->nh_pad)) ||
+      (rc = dalloc(c, &c->d_cls, (size_t)c->cls_cap)) ||
       (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) || (rc = dalloc(c, &c->d_vraw, mh)) ||
       (rc = dalloc(c, &c->d_vh, mh)) || (rc = dalloc(c, &c->d_vfull, 2 * mh)) ||
       (rc = dalloc(c, &c->d_keys, 2 * mh)) || (rc = dalloc(c, &c->d_keys_sorted, 2 * mh)) ||
@@ -414,16 +429,18 @@ int create_device(Ctx* c) {
     const size_t nq = (size_t)c->nq_pad;
     GroupSoA& g = c->gs;
     if ((rc = dalloc(c, &g.rec, nq)) || (rc = dalloc(c, &g.ant, 10 * nq)) ||
-        (rc = dalloc(c, &g.gia, nq)) || (rc = dalloc(c, &g.gib, nq)) || (rc = dalloc(c, &g.grep, nq)) ||
-        (rc = dalloc(c, &c->d_gcls_sky, (size_t)c->ntile * nq)) ||
-        (rc = dalloc(c, &c->d_gcls_img, (size_t)std::max<int64_t>(c->ngt, 1) * nq)))
+        (rc = dalloc(c, &g.gia, nq)) || (rc = dalloc(c, &g.gib, nq)) || (rc = dalloc(c, &g.grep, nq)))
       return rc;
-    size_t rp = 1;
+    int64_t rp = 1;
     for (int img = 0; img < 2; ++img) {
-      int64_t npts = img ? c->ngp : c->npix;
-      if (npts) rp = std::max(rp, (size_t)(recon_shape(npts).nchunk * npts));
+      const int64_t npts = img ? c->ngp : c->npix, nta = img ? c->ngt : c->ntile;
+      if (npts <= 0) continue;
+      const int64_t pp = recon_pass_tiles(c, nta, kind_w) * c->TP;
+      const int64_t last = npts - (npts - 1) / pp * pp;
+      for (int64_t n : {std::min(npts, pp), last}) rp = std::max(rp, recon_shape(n).nchunk * n);
     }
-    if ((rc = dalloc(c, &c->d_rpart, rp)) || (rc = dalloc(c, &c->d_rwpart, rp))) return rc;
+    c->rpart_cap = rp;
+    if ((rc = dalloc(c, &c->d_rpart, (size_t)rp)) || (rc = dalloc(c, &c->d_rwpart, (size_t)rp))) return rc;
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
@@ -452,14 +469,6 @@ int enq_gen(Ctx* c, int t0, int tlen) {
   k_seg_rows<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_pos, c->T, c->A, c->P, t0, c->nh, c->d_pa, c->d_pb, c->lambda,
                                                    c->d_rows);
   KC();
-  k_classify<<<blocks_for(c->ntile * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, c->d_rows, c->d_ant8, c->nh,
-                                                                  c->nh_pad, c->d_cls);
-  KC();
-  if (c->ngp && c->generic) {
-    k_classify<<<blocks_for(c->ngt * c->nh_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->d_rows, c->d_ant8, c->nh,
-                                                                  c->nh_pad, c->d_gcls);
-    KC();
-  }
   ev_end(c, ST_GEN);
   c->have_seg = true;
   c->have_viss = c->have_conj = c->have_groups = false;
@@ -473,36 +482,43 @@ void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
 
 int enq_viss(Ctx* c, bool count) {
   if (!c->have_seg || !c->have_sky) return c->fail(VW_ERR_STATE, "vw_viss_half needs vw_set_sky and vw_gen_segment");
-  VissShape s = viss_shape(c, c->nh);
-  if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
   if (count && !c->d_cpart) {
     int rc;
     if ((rc = dalloc(c, &c->d_cpart, (size_t)c->nchunk_max * c->nh_pad)) || (rc = dalloc(c, &c->d_counts, (size_t)c->max_nh)))
       return rc;
   }
-  VissArgs a{};
-  a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
-  a.npix = c->npix; a.ntile = c->ntile;
-  a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
-  a.nh = c->nh; a.nh_pad = c->nh_pad;
-  a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
-  a.part = c->d_part; a.cpart = c->d_cpart;
+  const bool mixed = c->cfg.precision == VW_PREC_MIXED;
   ev_begin(c, ST_VISS);
-  ev_begin(c, ST_VISSK);
-  unsigned nb = (unsigned)s.nblocks;
-  bool mixed = c->cfg.precision == VW_PREC_MIXED;
-  if (c->TP == 256) {
-    if (mixed) count ? launch_viss<256, 32, true, true>(a, nb, c->st) : launch_viss<256, 32, true, false>(a, nb, c->st);
-    else count ? launch_viss<256, 16, false, true>(a, nb, c->st) : launch_viss<256, 16, false, false>(a, nb, c->st);
-  } else {
-    if (mixed) count ? launch_viss<512, 32, true, true>(a, nb, c->st) : launch_viss<512, 32, true, false>(a, nb, c->st);
-    else count ? launch_viss<512, 16, false, true>(a, nb, c->st) : launch_viss<512, 16, false, false>(a, nb, c->st);
+  for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
+    const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
+    VissShape s = viss_shape(c, c->nh, nt);
+    if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
+    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, tile0, nt, c->d_rows, c->d_ant8,
+                                                              c->nh, c->nh_pad, c->d_cls);
+    KC();
+    VissArgs a{};
+    a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
+    a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
+    a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
+    a.nh = c->nh; a.nh_pad = c->nh_pad;
+    a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
+    a.part = c->d_part; a.cpart = c->d_cpart;
+    ev_begin(c, ST_VISSK);
+    unsigned nb = (unsigned)s.nblocks;
+    if (c->TP == 256) {
+      if (mixed) count ? launch_viss<256, 32, true, true>(a, nb, c->st) : launch_viss<256, 32, true, false>(a, nb, c->st);
+      else count ? launch_viss<256, 16, false, true>(a, nb, c->st) : launch_viss<256, 16, false, false>(a, nb, c->st);
+    } else {
+      if (mixed) count ? launch_viss<512, 32, true, true>(a, nb, c->st) : launch_viss<512, 32, true, false>(a, nb, c->st);
+      else count ? launch_viss<512, 16, false, true>(a, nb, c->st) : launch_viss<512, 16, false, false>(a, nb, c->st);
+    }
+    KC();
+    ev_end(c, ST_VISSK);
+    k_viss_reduce<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_part, s.nchunk, c->nh, c->nh_pad, tile0 > 0 ? 1 : 0,
+                                                        c->d_vraw, count ? c->d_cpart : nullptr,
+                                                        count ? c->d_counts : nullptr);
+    KC();
   }
-  KC();
-  ev_end(c, ST_VISSK);
-  k_viss_reduce<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_part, s.nchunk, c->nh, c->nh_pad, c->d_vraw,
-                                                      count ? c->d_cpart : nullptr, count ? c->d_counts : nullptr);
-  KC();
   ev_end(c, ST_VISS);
   c->have_viss = true;
   c->have_conj = c->have_groups = false;
@@ -545,19 +561,6 @@ int enq_group(Ctx* c) {
   k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
                                                   c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->d_info, c->gs);
   KC();
-  if (c->cfg.blockage && !c->generic) {
-    /* per-(tile, group) classes for the representative kernel */
-    if (c->cfg.healpix_out) {
-      k_group_cls_gather<<<blocks_for(c->ntile * c->nq_pad), 256, 0, c->st>>>(c->d_cls, c->ntile, c->nh_pad, c->gs.grep,
-                                                                              c->d_info, c->nq_pad, c->d_gcls_sky);
-      KC();
-    }
-    if (c->ngp) {
-      k_group_cls<<<blocks_for(c->ngt * c->nq_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->gs.gia, c->gs.gib, c->d_ant8,
-                                                                     c->d_info, c->nq_pad, c->d_gcls_img);
-      KC();
-    }
-  }
   ev_end(c, ST_GROUP);
   c->have_groups = true;
   return VW_OK;
@@ -570,53 +573,73 @@ void launch_recon(const ReconArgs& a, hipStream_t st) {
 }
 
 int enq_recon_set(Ctx* c, bool img) {
-  ReconArgs a{};
-  a.l = img ? c->d_gl : c->d_l;
-  a.m = img ? c->d_gm : c->d_m;
-  a.n = img ? c->d_gn : c->d_n;
-  a.npts = img ? c->ngp : c->npix;
-  a.TP = c->TP;
-  a.gcls = (const uint32_t*)(img ? c->d_gcls_img : c->d_gcls_sky);
-  a.nq_pad = c->nq_pad;
-  a.rcls = (const uint32_t*)(img ? c->d_gcls : c->d_cls);
-  a.nh_pad = c->nh_pad;
-  a.g = c->gs;
-  a.goff = c->d_goff;
-  a.idx_sorted = c->d_idx_sorted;
-  a.vh = c->d_vh;
-  a.rows = c->d_rows;
-  a.ant8 = c->d_ant8;
-  a.info = c->d_info;
-  a.P = c->P;
-  a.full = c->cfg.recon_mode == VW_RECON_FULL;
-  a.occ_mode = c->cfg.occ_mode;
-  ReconShape rs = recon_shape(a.npts);
-  a.pwaves = rs.pwaves;
-  a.nchunk = rs.nchunk;
-  a.pstride = a.npts;
-  a.part = c->d_rpart;
-  a.wpart = c->d_rwpart;
   int kind;
   if (!c->cfg.blockage) kind = VW_KIND_AVG;
   else if (!c->generic) kind = VW_KIND_REP;
   else kind = VW_KIND_GENERIC;
-  bool mixed = c->cfg.precision == VW_PREC_MIXED;
-  ev_begin(c, ST_RECONK);
-  if (mixed) {
-    if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, c->st);
-    else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, c->st);
-    else launch_recon<true, VW_KIND_AVG>(a, c->st);
-  } else {
-    if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, c->st);
-    else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, c->st);
-    else launch_recon<false, VW_KIND_AVG>(a, c->st);
+  const bool mixed = c->cfg.precision == VW_PREC_MIXED;
+  const int64_t npts_all = img ? c->ngp : c->npix;
+  const int64_t ntile_all = img ? c->ngt : c->ntile;
+  const double* cone = img ? c->d_gcone : c->d_cone;
+  const int64_t kind_w = kind == VW_KIND_GENERIC ? c->nh_pad : c->nq_pad;
+  const int64_t otpp = recon_pass_tiles(c, ntile_all, kind_w);
+  for (int64_t tile0 = 0; tile0 < ntile_all; tile0 += otpp) {
+    const int64_t nt = std::min(otpp, ntile_all - tile0);
+    const int64_t pt0 = tile0 * c->TP;
+    const int64_t np = std::min(nt * c->TP, npts_all - pt0);
+    if (kind == VW_KIND_REP) {
+      k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->gs.gia, c->gs.gib,
+                                                                 c->d_ant8, c->d_info, c->nq_pad, c->d_cls);
+      KC();
+    } else if (kind == VW_KIND_GENERIC) {
+      k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->d_rows, c->d_ant8, c->nh,
+                                                                c->nh_pad, c->d_cls);
+      KC();
+    }
+    ReconArgs a{};
+    a.l = img ? c->d_gl : c->d_l;
+    a.m = img ? c->d_gm : c->d_m;
+    a.n = img ? c->d_gn : c->d_n;
+    a.pt0 = pt0;
+    a.npts = np;
+    a.TP = c->TP;
+    a.gcls = (const uint32_t*)c->d_cls;
+    a.nq_pad = c->nq_pad;
+    a.rcls = (const uint32_t*)c->d_cls;
+    a.nh_pad = c->nh_pad;
+    a.g = c->gs;
+    a.goff = c->d_goff;
+    a.idx_sorted = c->d_idx_sorted;
+    a.vh = c->d_vh;
+    a.rows = c->d_rows;
+    a.ant8 = c->d_ant8;
+    a.info = c->d_info;
+    a.P = c->P;
+    a.full = c->cfg.recon_mode == VW_RECON_FULL;
+    a.occ_mode = c->cfg.occ_mode;
+    ReconShape rs = recon_shape(np);
+    if (rs.nchunk * np > c->rpart_cap) return c->fail(VW_ERR_RANGE, "internal: recon partial buffer too small");
+    a.pwaves = rs.pwaves;
+    a.nchunk = rs.nchunk;
+    a.pstride = np;
+    a.part = c->d_rpart;
+    a.wpart = c->d_rwpart;
+    ev_begin(c, ST_RECONK);
+    if (mixed) {
+      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, c->st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, c->st);
+      else launch_recon<true, VW_KIND_AVG>(a, c->st);
+    } else {
+      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, c->st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, c->st);
+      else launch_recon<false, VW_KIND_AVG>(a, c->st);
+    }
+    KC();
+    ev_end(c, ST_RECONK);
+    k_recon_reduce<<<blocks_for(np), 256, 0, c->st>>>(a.part, a.wpart, a.nchunk, a.pstride, pt0, np, c->cfg.blockage,
+                                                      img ? c->d_Gacc : c->d_Cacc, img ? c->d_GWacc : c->d_Wacc);
+    KC();
   }
-  KC();
-  ev_end(c, ST_RECONK);
-  k_recon_reduce<<<blocks_for(a.npts), 256, 0, c->st>>>(a.part, a.wpart, a.nchunk, a.pstride, a.npts,
-                                                        c->cfg.blockage, img ? c->d_Gacc : c->d_Cacc,
-                                                        img ? c->d_GWacc : c->d_Wacc);
-  KC();
   return VW_OK;
 }
 
@@ -971,27 +994,35 @@ VW_API int vw_get_rows(void* p, double* u, double* v, double* w) {
   return VW_OK;
 }
 
+namespace {
+/* full class table of one tile set for the current segment, on demand */
+int full_classes(Ctx* c, bool img, std::vector<uint8_t>& t, uint8_t** dev_out) {
+  int64_t nt = img ? c->ngt : c->ntile;
+  t.assign((size_t)(nt * c->nh_pad), 0);
+  uint8_t* d = nullptr;
+  HC(hipMalloc(&d, t.size()));
+  k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(img ? c->d_gcone : c->d_cone, nt, 0, nt, c->d_rows,
+                                                            c->d_ant8, c->nh, c->nh_pad, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(t.data(), d, t.size(), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess || !dev_out) (void)hipFree(d);
+  if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("class table: ") + hipGetErrorString(e));
+  if (dev_out) *dev_out = d;
+  return VW_OK;
+}
+
+}  // namespace
+
 VW_API int vw_get_tile_classes(void* p, int32_t img, uint8_t* cls) {
   Ctx* c = as_ctx(p);
   if (!c || !cls) return VW_ERR_ARG;
   if (!c->have_seg) return c->fail(VW_ERR_STATE, "no segment");
   if (img && !c->ngp) return c->fail(VW_ERR_ARG, "no lm grid");
   int64_t nt = img ? c->ngt : c->ntile;
-  std::vector<uint8_t> t((size_t)(nt * c->nh_pad));
-  if (img && !c->generic) {
-    /* the image row classes are only kept for the generic recon; build them */
-    uint8_t* d = nullptr;
-    HC(hipMalloc(&d, t.size()));
-    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_gcone, c->ngt, c->d_rows, c->d_ant8, c->nh,
-                                                              c->nh_pad, d);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(t.data(), d, t.size(), hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    (void)hipFree(d);
-    if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("vw_get_tile_classes: ") + hipGetErrorString(e));
-  } else {
-    HC(hipMemcpy(t.data(), img ? c->d_gcls : c->d_cls, t.size(), hipMemcpyDeviceToHost));
-  }
+  std::vector<uint8_t> t;
+  int rc = full_classes(c, img != 0, t, nullptr);
+  if (rc) return rc;
   for (int64_t h = 0; h < c->nh; ++h)
     for (int64_t k = 0; k < nt; ++k) cls[h * nt + k] = t[k * c->nh_pad + h];
   return VW_OK;
@@ -1002,15 +1033,22 @@ VW_API int vw_get_masks(void* p, int64_t row0, int64_t nrows, uint8_t* mask) {
   if (!c || !mask) return VW_ERR_ARG;
   if (!c->have_seg) return c->fail(VW_ERR_STATE, "no segment");
   if (row0 < 0 || nrows < 0 || row0 + nrows > c->nh) return c->fail(VW_ERR_ARG, "row range");
+  std::vector<uint8_t> t;
+  uint8_t* dcls = nullptr;
+  int rc = full_classes(c, false, t, &dcls);
+  if (rc) return rc;
   int64_t tot = nrows * c->npix;
   uint8_t* d = nullptr;
-  HC(hipMalloc(&d, tot ? tot : 1));
-  k_masks<<<blocks_for(tot), 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->TP, c->d_cls, c->nh_pad, c->d_rows,
-                                              c->d_ant8, row0, nrows, d);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipMalloc(&d, tot ? tot : 1);
+  if (e == hipSuccess) {
+    k_masks<<<blocks_for(tot), 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->TP, dcls, c->nh_pad, c->d_rows,
+                                                c->d_ant8, row0, nrows, d);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(mask, d, tot, hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
   (void)hipFree(d);
+  (void)hipFree(dcls);
   if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("vw_get_masks: ") + hipGetErrorString(e));
   return VW_OK;
 }

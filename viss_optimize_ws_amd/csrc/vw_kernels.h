@@ -240,14 +240,15 @@ __global__ void k_seg_full(const double* __restrict__ pos, int T, int A, int P, 
   out[8 * nf + i] = pos[2 * TA + k2];
 }
 
-/* pair class per (tile, half row) -> bytes [ntile][nh_pad] */
-__global__ void k_classify(const double* __restrict__ cone, int64_t ntile, const RowRec* __restrict__ rows,
-                           const double* __restrict__ ant8, int64_t nh, int64_t nh_pad,
-                           uint8_t* __restrict__ cls) {
+/* pair class per (tile, half row) for tiles [tile0, tile0+nt) -> bytes
+ * [nt][nh_pad]; cone arrays hold all ntile tiles */
+__global__ void k_classify(const double* __restrict__ cone, int64_t ntile, int64_t tile0, int64_t nt,
+                           const RowRec* __restrict__ rows, const double* __restrict__ ant8, int64_t nh,
+                           int64_t nh_pad, uint8_t* __restrict__ cls) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = ntile * nh_pad;
-  if (k >= total) return;
-  int64_t t = k / nh_pad, h = k - t * nh_pad;
+  if (k >= nt * nh_pad) return;
+  int64_t tr = k / nh_pad, h = k - tr * nh_pad;
+  int64_t t = tile0 + tr;
   if (h >= nh) { cls[k] = VW_CLS_HIDDEN; return; }
   RowRec r = rows[h];
   const double* qa = ant8 + (int64_t)r.ia * 8;
@@ -288,10 +289,11 @@ __global__ void k_masks(const double* __restrict__ l, const double* __restrict__
 struct VissArgs {
   const double *l, *m, *n, *nm1, *B;
   const float* Bf;
-  int64_t npix, ntile;
+  int64_t npix;
+  int64_t tile0, ntile; /* tiles of this pass: [tile0, tile0 + ntile) */
   const RowRec* rows;
   const double* ant8;
-  const uint32_t* cls; /* [ntile][nh_pad] bytes viewed as words */
+  const uint32_t* cls; /* [ntile][nh_pad] bytes (pass-relative) viewed as words */
   int64_t nh, nh_pad;
   int64_t tpc, nchunk, nrb, bpc;
   double2* part; /* [nchunk][nh_pad] */
@@ -307,7 +309,8 @@ __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
  * applied when the partial is written) */
 template <bool MIXED>
 __device__ __forceinline__ void vw_viss_term(const RowRec& rec, double l, double m, double nm1, double bd, float bf,
-                                             float& fre, float& fim, double& dre, double& dim) {
+                                             const double* tab, float& fre, float& fim, double& dre,
+                                             double& dim) {
   const double x = vw_viss_phase(rec.u, rec.v, rec.w, l, m, nm1);
   if (MIXED) {
     const float f = (float)(x - rint(x));
@@ -317,7 +320,7 @@ __device__ __forceinline__ void vw_viss_term(const RowRec& rec, double l, double
     fim = fmaf(bf, s, fim);
   } else {
     double s, c;
-    vw_sincos_turn(x, &s, &c);
+    vw_sincos_turn_tab(x, tab, &s, &c);
     dre = fma(bd, c, dre);
     dim = fma(bd, s, dim);
   }
@@ -326,14 +329,20 @@ __device__ __forceinline__ void vw_viss_term(const RowRec& rec, double l, double
 template <int TP, int RPW, bool MIXED, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_viss(VissArgs a) {
   constexpr int PPL = TP / 64;
+  __shared__ double tab[2 * VW_TAB_N];
+  if (!MIXED) {
+    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, tab);
+    __syncthreads();
+  }
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t chunk = vb / a.bpc;
   const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int64_t rb = (vb - chunk * a.bpc) * 4 + wave;
   if (chunk >= a.nchunk || rb >= a.nrb) return;
   const int64_t r0 = rb * RPW;
-  const int64_t t0 = chunk * a.tpc;
-  const int64_t t1 = (t0 + a.tpc < a.ntile) ? t0 + a.tpc : a.ntile;
+  const int64_t t0 = a.tile0 + chunk * a.tpc;
+  const int64_t tend = a.tile0 + a.ntile;
+  const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
 
   float fre[RPW], fim[RPW];
   double dre[RPW], dim[RPW];
@@ -362,13 +371,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int r = 0; r < RPW; ++r) {
       const int64_t h = r0 + r;
       if (h < a.nh) {
-        const int cls = vw_cls_lookup(a.cls, tile, a.nh_pad, h);
+        const int cls = vw_cls_lookup(a.cls, tile - a.tile0, a.nh_pad, h);
         if (cls == VW_CLS_VISIBLE) {
           const RowRec rec = a.rows[h];
 #pragma unroll
           for (int k = 0; k < PPL; ++k) {
-            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], MIXED ? 0.0 : pb[k], MIXED ? pbf[k] : 0.f, fre[r], fim[r],
-                                dre[r], dim[r]);
+            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], MIXED ? 0.0 : pb[k], MIXED ? pbf[k] : 0.f, tab, fre[r],
+                                fim[r], dre[r], dim[r]);
             if (COUNT) cnt[r] += ok[k] ? 1 : 0;
           }
         } else if (cls == VW_CLS_MIXED) {
@@ -381,8 +390,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           for (int k = 0; k < PPL; ++k) {
             const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
                              vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
-            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], vis ? pb[k] : 0.0, vis ? pbf[k] : 0.f, fre[r], fim[r],
-                                dre[r], dim[r]);
+            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], vis ? pb[k] : 0.0, vis ? pbf[k] : 0.f, tab, fre[r],
+                                fim[r], dre[r], dim[r]);
             if (COUNT) cnt[r] += vis ? 1 : 0;
           }
         }
@@ -406,9 +415,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
-/* partials -> raw half sums, chunks folded in ascending order */
+/* partials -> raw half sums, chunks folded in ascending order; passes after
+ * the first add to the running sums */
 __global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, int64_t nh, int64_t nh_pad,
-                              double2* __restrict__ vraw, const int64_t* __restrict__ cpart,
+                              int accumulate, double2* __restrict__ vraw, const int64_t* __restrict__ cpart,
                               int64_t* __restrict__ counts) {
   int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
@@ -419,6 +429,12 @@ __global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, 
     re += v.x;
     im += v.y;
     if (counts) c += cpart[k * nh_pad + h];
+  }
+  if (accumulate) {
+    double2 o = vraw[h];
+    re = o.x + re;
+    im = o.y + im;
+    if (counts) c += counts[h];
   }
   vraw[h] = make_double2(re, im);
   if (counts) counts[h] = c;
@@ -661,23 +677,15 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
   d[5] = qb[0]; d[6] = qb[1]; d[7] = qb[2]; d[8] = qb[3]; d[9] = qb[7];
 }
 
-/* class of (output tile, group) = class of (tile, representative row) */
-__global__ void k_group_cls_gather(const uint8_t* __restrict__ rcls, int64_t ntile, int64_t nh_pad,
-                                   const int32_t* __restrict__ grep, const DevInfo* __restrict__ info,
-                                   int64_t nq_pad, uint8_t* __restrict__ gcls) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ntile * nq_pad) return;
-  int64_t t = k / nq_pad, q = k - t * nq_pad;
-  gcls[k] = q < info->nuniq ? rcls[t * nh_pad + grep[q]] : (uint8_t)VW_CLS_HIDDEN;
-}
-
 /* class of (output tile, group) from the representative's antennas */
-__global__ void k_group_cls(const double* __restrict__ cone, int64_t ntile, const int32_t* __restrict__ gia,
-                            const int32_t* __restrict__ gib, const double* __restrict__ ant8,
-                            const DevInfo* __restrict__ info, int64_t nq_pad, uint8_t* __restrict__ gcls) {
+__global__ void k_group_cls(const double* __restrict__ cone, int64_t ntile, int64_t tile0, int64_t nt,
+                            const int32_t* __restrict__ gia, const int32_t* __restrict__ gib,
+                            const double* __restrict__ ant8, const DevInfo* __restrict__ info, int64_t nq_pad,
+                            uint8_t* __restrict__ gcls) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ntile * nq_pad) return;
-  int64_t t = k / nq_pad, q = k - t * nq_pad;
+  if (k >= nt * nq_pad) return;
+  int64_t tr = k / nq_pad, q = k - tr * nq_pad;
+  int64_t t = tile0 + tr;
   if (q >= info->nuniq) { gcls[k] = VW_CLS_HIDDEN; return; }
   const double* qa = ant8 + (int64_t)gia[q] * 8;
   const double* qb = ant8 + (int64_t)gib[q] * 8;
@@ -693,7 +701,7 @@ __global__ void k_group_cls(const double* __restrict__ cone, int64_t ntile, cons
 /* ------------------------------------------------------------------ */
 struct ReconArgs {
   const double *l, *m, *n;
-  int64_t npts, TP;
+  int64_t pt0, npts, TP; /* points [pt0, pt0 + npts) of this pass, pt0 = tile0 * TP */
   const uint32_t* gcls; /* [ntile_out][nq_pad]: per group (representative kind) */
   int64_t nq_pad;
   const uint32_t* rcls; /* [ntile_out][nh_pad]: per half row (generic kind)     */
@@ -737,8 +745,9 @@ struct ReconAcc;
 template <>
 struct ReconAcc<false> {
   double a = 0.0;
-  __device__ void term(double ugu, double vgu, double vr, double vi, float, float, double lp, double mp) {
-    a = a + vw_recon_term(vr, vi, vw_recon_phase(ugu, vgu, lp, mp));
+  __device__ void term(double ugu, double vgu, double vr, double vi, float, float, double lp, double mp,
+                       const double* tab) {
+    a = a + vw_recon_term(vr, vi, vw_recon_phase(ugu, vgu, lp, mp), tab);
   }
   __device__ void fold() {}
   __device__ double value() const { return a; }
@@ -749,7 +758,8 @@ struct ReconAcc<false> {
 template <>
 struct ReconAcc<true> {
   float s = 0.f, c = 0.f, b = 0.f;
-  __device__ void term(double ugu, double vgu, double, double, float vrf, float vif, double lp, double mp) {
+  __device__ void term(double ugu, double vgu, double, double, float vrf, float vif, double lp, double mp,
+                       const double*) {
     const double x = vw_recon_phase(ugu, vgu, lp, mp);
     const float f = (float)(x - rint(x));
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
@@ -767,13 +777,18 @@ struct ReconAcc<true> {
 
 template <int PPL, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
+  __shared__ double tab[2 * VW_TAB_N];
+  if (!MIXED) {
+    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, tab);
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * 4 + vw_rfl((int)(threadIdx.x >> 6));
   const int64_t chunk = wid / a.pwaves;
   const int64_t pw = wid - chunk * a.pwaves;
   if (chunk >= a.nchunk) return;
-  const int64_t pbase = pw * 64 * PPL;
-  const int64_t tile = pbase / a.TP;
+  const int64_t pbase = pw * 64 * PPL; /* pass-relative */
+  const int64_t tile = pbase / a.TP;   /* pass-relative tile (class tables are per pass) */
   const vw_segparams sp = a.info->sp;
   const int64_t nu = a.info->nuniq;
   const int64_t q0 = a.info->first_nz;
@@ -790,7 +805,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   for (int k = 0; k < PPL; ++k) {
     int64_t p = pbase + lane + 64 * k;
     ok[k] = p < a.npts;
-    int64_t pp = ok[k] ? p : 0;
+    int64_t pp = a.pt0 + (ok[k] ? p : 0);
     pl[k] = a.l[pp];
     pm[k] = a.m[pp];
     pn[k] = a.n[pp];
@@ -833,7 +848,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
         const float vrf = (float)g.vr, vif = (float)g.vi;
         if (cls == VW_CLS_VISIBLE) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k]);
+          for (int k = 0; k < PPL; ++k) acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k], tab);
           wu += 1;
         } else {
           const double* d = a.g.ant + q * 10;
@@ -844,7 +859,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
             const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
                              vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
             if (vis) {
-              acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k]);
+              acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k], tab);
               wl[k] += 1;
             }
           }
@@ -898,7 +913,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
         if (cn[k] > 0) {
           const double invc = 1.0 / (double)cn[k];
           const double ar = sr[k] * invc, ai = si[k] * invc;
-          acc[k].term(ugu, vgu, ar, ai, (float)ar, (float)ai, lp[k], mp[k]);
+          acc[k].term(ugu, vgu, ar, ai, (float)ar, (float)ai, lp[k], mp[k], tab);
           wl[k] += 1;
         }
       }
@@ -918,9 +933,10 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   }
 }
 
-/* fold group-range partials in order into the accumulators (add_inplace) */
+/* fold group-range partials in order into the accumulators (add_inplace) for
+ * points [pt0, pt0 + npts) */
 __global__ void k_recon_reduce(const double* __restrict__ part, const uint32_t* __restrict__ wpart, int64_t nchunk,
-                               int64_t pstride, int64_t npts, int weights, double* __restrict__ acc,
+                               int64_t pstride, int64_t pt0, int64_t npts, int weights, double* __restrict__ acc,
                                uint32_t* __restrict__ wacc) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
@@ -930,8 +946,8 @@ __global__ void k_recon_reduce(const double* __restrict__ part, const uint32_t* 
     s = s + part[k * pstride + p];
     w += wpart[k * pstride + p];
   }
-  acc[p] = acc[p] + s;
-  if (weights) wacc[p] += w;
+  acc[pt0 + p] = acc[pt0 + p] + s;
+  if (weights) wacc[pt0 + p] += w;
 }
 
 /* final maps: C = acc / W (blockage 1) or acc * (1/W_total) (blockage 0) */

@@ -94,6 +94,34 @@ VW_HD void vw_sincos_turn(double r, double* s_out, double* c_out) {
   *c_out = ((qi + 1) & 2) ? -b : b;
 }
 
+/* Table-driven sin/cos of 2*pi*x (Tang's method): with r = x - rint(x),
+ * r*256 = k + f exactly with |f| <= 1/2, so 2*pi*x = 2*pi*k/256 + y with |y| <= pi/256; sin/cos(y) by
+ * short Taylor polynomials (omitted terms < 2e-20), then one rotation by the
+ * table entry (cos, sin)(2*pi*k/256).  The table is built by
+ * vw_sincos_tab_build with vw_sincos_turn, identically on host and device.
+ * Error within ~2 ulp of sin/cos; about 20 operations plus one table read. */
+#define VW_TAB_N 256
+VW_HD void vw_sincos_tab_build(int k, double* tab) {
+  vw_sincos_turn((double)k / (double)VW_TAB_N, &tab[2 * k + 1], &tab[2 * k]);
+}
+
+VW_HD void vw_sincos_turn_tab(double x, const double* tab, double* s_out, double* c_out) {
+  double r = x - rint(x);           /* exact, |r| <= 1/2 turn */
+  double t = r * 256.0;             /* exact */
+  double k = rint(t);               /* -128 .. 128 */
+  double f = t - k;                 /* exact, |f| <= 1/2 */
+  double y = fma(f, 0x1.921fb54442d18p-6, f * 0x1.1a62633145c07p-60); /* f * 2 pi / 256 */
+  double z = y * y;
+  double ps = fma(fma(-0x1.a01a01a01a01ap-13, z, 0x1.1111111111111p-7), z, -0x1.5555555555555p-3);
+  double sy = fma(y * z, ps, y);
+  double pc = fma(fma(-0x1.6c16c16c16c17p-10, z, 0x1.5555555555555p-5), z, -0.5);
+  double cy = fma(z, pc, 1.0);
+  int ki = ((int)k) & (VW_TAB_N - 1);
+  double tc = tab[2 * ki], ts = tab[2 * ki + 1];
+  *s_out = fma(ts, cy, tc * sy);
+  *c_out = fma(tc, cy, -(ts * sy));
+}
+
 VW_HD double vw_dot3(double ax, double ay, double az, double bx, double by, double bz) {
   return fma(az, bz, fma(ay, by, ax * bx));
 }
@@ -453,9 +481,9 @@ VW_HD double vw_recon_phase(double ugu, double vgu, double lp, double mp) {
 }
 
 /* Re(V exp(+j 2 pi x)) = Vr cos - Vi sin */
-VW_HD double vw_recon_term(double vr, double vi, double x) {
+VW_HD double vw_recon_term(double vr, double vi, double x, const double* tab) {
   double s, c;
-  vw_sincos_turn(x, &s, &c);
+  vw_sincos_turn_tab(x, tab, &s, &c);
   return fma(vr, c, -(vi * s));
 }
 
