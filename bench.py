@@ -32,11 +32,13 @@ WORKLOAD_NAME = ("HEALPix NESTED Nside=128 (196,608 px, 768 tiles), 16 antennas 
                  "circular orbits r in [6900,7100] km, 512 epochs x 10 s, 30 MHz, 16 segments, "
                  "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 512x512 lm image")
 FP64_PEAK_TFLOPS = 78.6
-# FP64 flops of one evaluated Viss term in the plain form (shared-header
-# formulas): phase 1 mul + 2 fma = 5; turn reduction 3; 2*pi scaling
-# fma + mul = 3; y^2 1; sin poly 8 fma + mul + fma = 19; cos poly 9 fma = 18;
-# accumulation 2 fma = 4.
-VISS_TERM_FLOPS_PLAIN = 53
+# FP64 flops of one evaluated Viss term in the plain form as executed (fma =
+# 2; rint and the table index are not counted): phase u*l + v*m + w*(n-1)
+# 1 mul + 2 fma = 5; reduction x - rint(x), *256, - k = 3; y = f*2pi/256 in
+# two parts 1 mul + 1 fma = 3; z = y^2 1; sin poly 2 fma + mul + fma = 7;
+# cos poly 3 fma = 6; angle addition 2 mul + 2 fma = 6; accumulation
+# 2 fma = 4.
+VISS_TERM_FLOPS_PLAIN = 35
 
 
 def _cfg(precision):
@@ -104,6 +106,22 @@ def _cpu_baseline(sky, threads):
                 sample=f"segment 1 of 16 in full: {terms:.3e} terms in {dt:.2f} s (oracle, OpenMP FP64)")
 
 
+def _copy_calibration(mib):
+    """A plain copy kernel of known size: n bytes read and n bytes written per
+    launch.  Its FETCH_SIZE / WRITE_SIZE readings give the factor that turns
+    those counters into bytes for the kernels of the path."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel, not a DMA copy
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +129,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calib:
+        _copy_calibration(args.copy_calib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one MI355X)",
               file=sys.stderr)

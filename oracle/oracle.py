@@ -31,6 +31,8 @@ def load():
         lib.vo_mt64_seed.restype = None
         lib.vo_mt64_next.argtypes = [C.c_void_p]
         lib.vo_mt64_next.restype = C.c_uint64
+        lib.vo_sincos.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
+        lib.vo_sincos.restype = None
         _LIB = lib
     return _LIB
 
@@ -68,6 +70,17 @@ class Oracle(Stages):
         out = np.zeros((nrows, self.plan.npix), np.uint8)
         self._check(self._lib.vo_get_predicate(self._ctx, int(row0), int(nrows), _ptr(out, C.c_uint8)))
         return out
+
+
+def sincos_turn(x, form):
+    """sin and cos of 2 pi x by the oracle's shared formulas (form 0: exact
+    reduction + Taylor; form 1: 1024-entry table)."""
+    x = np.ascontiguousarray(x, np.float64)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    load().vo_sincos(_ptr(x, C.c_double), C.c_int64(x.size), C.c_int32(form), _ptr(s, C.c_double),
+                     _ptr(c, C.c_double))
+    return s, c
 
 
 class MT64:

@@ -113,7 +113,7 @@ const char* vo_last_error(void* ctx) { return ctx ? ((vo_ctx*)ctx)->err : g_crea
 
 static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
-/* table of (cos, sin)(2 pi k / 256) for the table-driven term sincos */
+/* table of (cos, sin)(2 pi k / VW_TAB_N) for the table-driven term sincos */
 static double g_tab[2 * VW_TAB_N];
 static void build_tab(void) {
   for (int k = 0; k < VW_TAB_N; ++k) vw_sincos_tab_build(k, g_tab);
@@ -490,7 +490,7 @@ int vo_viss_half(void* ctx, int64_t row0, int64_t nrows, vw_c128* out, int64_t* 
       for (int64_t p = t * c->TP; p < p1; ++p) {
         if (cls == VW_CLS_MIXED && !predicate(c, h, c->l[p], c->m[p], c->n[p])) continue;
         double s, co;
-        vw_sincos_turn_tab(vw_viss_phase(u, v, w, c->l[p], c->m[p], c->nm1[p]), g_tab, &s, &co);
+        vw_sincos_tab(vw_viss_phase(u, v, w, c->l[p], c->m[p], c->nm1[p]) * VW_TAB_SCALE, g_tab, &s, &co);
         re = re + c->B[p] * co;
         im = im - c->B[p] * s;
         ++cnt;
@@ -631,16 +631,19 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
     if (key == 0) continue;
     double ugu, vgu;
     vw_key_uv(key, &c->sp, &ugu, &vgu);
-    double x = vw_recon_phase(ugu, vgu, lp, mp);
+    const double t = vw_recon_phase(ugu, vgu, lp, mp) * VW_TAB_SCALE;
+    double sx, cx; /* sin, cos of the phase, evaluated where the term is used */
     if (!c->cfg.blockage) {
-      acc = acc + vw_recon_term(c->gavg[q].re, c->gavg[q].im, x, g_tab);
+      vw_sincos_tab(t, g_tab, &sx, &cx);
+      acc = vw_recon_add(acc, c->gavg[q].re, c->gavg[q].im, sx, cx);
       continue;
     }
     int32_t s0 = c->goff[q], s1 = c->goff[q + 1], L = s1 - s0;
     if (mirror && c->cfg.occ_mode == 2) {
       int64_t h = c->grep[q];
       if (eff_mask(c, h, img, pt)) {
-        acc = acc + vw_recon_term(c->vh[h].re, c->vh[h].im, x, g_tab);
+        vw_sincos_tab(t, g_tab, &sx, &cx);
+        acc = vw_recon_add(acc, c->vh[h].re, c->vh[h].im, sx, cx);
         w += 1;
       }
       continue;
@@ -664,7 +667,8 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
     }
     if (cnt > 0) {
       double invc = 1.0 / (double)cnt;
-      acc = acc + vw_recon_term(sr * invc, si * invc, x, g_tab);
+      vw_sincos_tab(t, g_tab, &sx, &cx);
+      acc = vw_recon_add(acc, sr * invc, si * invc, sx, cx);
       w += 1;
     }
   }
@@ -836,4 +840,12 @@ int vo_get_groups(void* ctx, vw_seg_info* info, int64_t* keys, int32_t* offsets,
   if (idx_sorted) memcpy(idx_sorted, c->idx_sorted, sizeof(int32_t) * (size_t)c->nrows_grp);
   if (rep) memcpy(rep, c->grep, sizeof(int32_t) * (size_t)c->nuniq);
   return VW_OK;
+}
+
+void vo_sincos(const double* x, int64_t n, int32_t form, double* s, double* c) {
+  build_tab();
+  for (int64_t i = 0; i < n; ++i) {
+    if (form == 0) vw_sincos_turn(x[i], &s[i], &c[i]);
+    else vw_sincos_tab(x[i] * VW_TAB_SCALE, g_tab, &s[i], &c[i]);
+  }
 }

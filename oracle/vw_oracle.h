@@ -63,6 +63,9 @@ int vo_get_groups(void* ctx, vw_seg_info* info, int64_t* keys, int32_t* offsets,
                   int32_t* idx_sorted, int32_t* rep);
 /* Pure per-pixel occlusion predicate (no tile cones), [nrows][npix]. */
 int vo_get_predicate(void* ctx, int64_t row0, int64_t nrows, uint8_t* mask);
+/* sin and cos of 2 pi x[i] (x in turns): form 0 = exact reduction + Taylor
+ * (vw_sincos_turn), form 1 = table-driven (vw_sincos_tab of 1024 x). */
+void vo_sincos(const double* x, int64_t n, int32_t form, double* s, double* c);
 
 #ifdef __cplusplus
 }

@@ -215,3 +215,27 @@ def test_equal_split_schedule(T, segs):
             want.append((cur, ln))
             cur += ln
         assert o.schedule() == want
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_sincos_turn_accuracy(form):
+    # both sine/cosine forms of the shared header against long-double numpy
+    # (x - round(x) is exact, so the reference is accurate to ~1e-19)
+    from oracle.oracle import sincos_turn
+
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-1, 1, 20000), rng.uniform(-2e6, 2e6, 20000),
+                        rng.integers(-4096, 4096, 2000) / 1024.0 + rng.uniform(-1e-3, 1e-3, 2000) / 1024.0,
+                        (rng.integers(-4096, 4096, 1000) + 0.5) / 1024.0])
+    s, c = sincos_turn(x, form)
+    r = (x - np.round(x)).astype(np.longdouble)
+    two_pi = np.longdouble(2) * np.arccos(np.longdouble(-1))
+    es = np.abs(s - np.sin(two_pi * r).astype(np.float64)).max()
+    ec = np.abs(c - np.cos(two_pi * r).astype(np.float64)).max()
+    assert es <= 3.4e-16 and ec <= 3.4e-16, (es, ec)
+    # exact quarter turns and the Pythagorean identity
+    q = np.array([0.0, 0.25, 0.5, 0.75, 1.0, -0.25, 1e6 + 0.5])
+    sq, cq = sincos_turn(q, form)
+    assert np.array_equal(sq, [0.0, 1.0, 0.0, -1.0, 0.0, -1.0, 0.0])
+    assert np.array_equal(cq, [1.0, 0.0, -1.0, 0.0, 1.0, 0.0, -1.0])
+    assert np.abs(s * s + c * c - 1).max() <= 6e-16

@@ -60,6 +60,7 @@ struct Ctx {
   double* d_circ = nullptr;
   int32_t *d_pa = nullptr, *d_pb = nullptr;
   RowRec* d_rows = nullptr;
+  double4* d_rowt = nullptr; /* half-row (u, v, w) * 1024 for the plain Viss phase */
   double* d_ant8 = nullptr;
   uint8_t* d_cls = nullptr; /* class scratch shared by all tile passes (cls_cap bytes) */
   int64_t cls_cap = 0;       /* bytes                                                    */
@@ -265,10 +266,16 @@ struct VissShape {
   int rpw;
   int64_t tpc, nchunk, nrb, bpc, nblocks;
 };
-VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt) {
+/* half rows per wave: as many accumulators as fit without spilling
+ * (the counting variants carry one more register per row) */
+constexpr int viss_rpw(int /*TP*/, bool mixed, bool count) {
+  return (mixed ? 32 : 16) / (count ? 2 : 1);
+}
+
+VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {
   VissShape s;
   bool mixed = c->cfg.precision == VW_PREC_MIXED;
-  s.rpw = mixed ? 32 : 16;
+  s.rpw = viss_rpw(c->TP, mixed, count);
   s.nrb = (nh + s.rpw - 1) / s.rpw;
   const int64_t target_waves = 8192;
   int64_t nchunk = (target_waves + s.nrb - 1) / s.nrb;
@@ -413,7 +420,7 @@ int create_device(Ctx* c) {
   for (int k = 0; k < c->segs; ++k)
     for (int64_t nt : {c->vtpp, vlast}) nch = std::max(nch, viss_shape(c, (int64_t)c->seg_tlen[k] * c->P, nt).nchunk);
   c->nchunk_max = nch;
-  if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
+  if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_rowt, (size_t)c->nh_pad)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
       (rc = dalloc(c, &c->d_cls, (size_t)c->cls_cap)) ||
       (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) || (rc = dalloc(c, &c->d_vraw, mh)) ||
       (rc = dalloc(c, &c->d_vh, mh)) || (rc = dalloc(c, &c->d_vfull, 2 * mh)) ||
@@ -467,7 +474,7 @@ int enq_gen(Ctx* c, int t0, int tlen) {
                                                c->cosphi_glob, c->d_ant8);
   KC();
   k_seg_rows<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_pos, c->T, c->A, c->P, t0, c->nh, c->d_pa, c->d_pb, c->lambda,
-                                                   c->d_rows);
+                                                   c->d_rows, c->d_rowt);
   KC();
   ev_end(c, ST_GEN);
   c->have_seg = true;
@@ -491,7 +498,7 @@ int enq_viss(Ctx* c, bool count) {
   ev_begin(c, ST_VISS);
   for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
-    VissShape s = viss_shape(c, c->nh, nt);
+    VissShape s = viss_shape(c, c->nh, nt, count);
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
     k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, tile0, nt, c->d_rows, c->d_ant8,
                                                               c->nh, c->nh_pad, c->d_cls);
@@ -499,19 +506,21 @@ int enq_viss(Ctx* c, bool count) {
     VissArgs a{};
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
-    a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
+    a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
     a.nh = c->nh; a.nh_pad = c->nh_pad;
     a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
     a.part = c->d_part; a.cpart = c->d_cpart;
     ev_begin(c, ST_VISSK);
     unsigned nb = (unsigned)s.nblocks;
+#define VW_VISS(TP_, M_, C_) launch_viss<TP_, viss_rpw(TP_, M_, C_), M_, C_>(a, nb, c->st)
     if (c->TP == 256) {
-      if (mixed) count ? launch_viss<256, 32, true, true>(a, nb, c->st) : launch_viss<256, 32, true, false>(a, nb, c->st);
-      else count ? launch_viss<256, 16, false, true>(a, nb, c->st) : launch_viss<256, 16, false, false>(a, nb, c->st);
+      if (mixed) count ? VW_VISS(256, true, true) : VW_VISS(256, true, false);
+      else count ? VW_VISS(256, false, true) : VW_VISS(256, false, false);
     } else {
-      if (mixed) count ? launch_viss<512, 32, true, true>(a, nb, c->st) : launch_viss<512, 32, true, false>(a, nb, c->st);
-      else count ? launch_viss<512, 16, false, true>(a, nb, c->st) : launch_viss<512, 16, false, false>(a, nb, c->st);
+      if (mixed) count ? VW_VISS(512, true, true) : VW_VISS(512, true, false);
+      else count ? VW_VISS(512, false, true) : VW_VISS(512, false, false);
     }
+#undef VW_VISS
     KC();
     ev_end(c, ST_VISSK);
     k_viss_reduce<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_part, s.nchunk, c->nh, c->nh_pad, tile0 > 0 ? 1 : 0,
@@ -559,7 +568,9 @@ int enq_group(Ctx* c) {
   k_groups<<<1, 1024, 0, c->st>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
   KC();
   k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
-                                                  c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->d_info, c->gs);
+                                                  c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage,
+                                                  c->cfg.precision == VW_PREC_MIXED ? 1.0 : VW_TAB_SCALE, c->d_info,
+                                                  c->gs);
   KC();
   ev_end(c, ST_GROUP);
   c->have_groups = true;

@@ -197,7 +197,7 @@ __global__ void k_seg_ant(const double* __restrict__ pos, int T, int A, int t0, 
 
 __global__ void k_seg_rows(const double* __restrict__ pos, int T, int A, int P, int t0, int64_t nh,
                            const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, double lambda,
-                           RowRec* __restrict__ rows) {
+                           RowRec* __restrict__ rows, double4* __restrict__ rowt) {
   int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   int64_t TA = (int64_t)T * A;
@@ -211,6 +211,8 @@ __global__ void k_seg_rows(const double* __restrict__ pos, int T, int A, int P, 
   r.ia = (int32_t)(g * A + pa[j]);
   r.ib = (int32_t)(g * A + pb[j]);
   rows[h] = r;
+  /* the plain Viss phase in 1/1024 turns (exact power-of-two scaling) */
+  rowt[h] = make_double4(r.u * VW_TAB_SCALE, r.v * VW_TAB_SCALE, r.w * VW_TAB_SCALE, 0.0);
 }
 
 /* reference full-row layout (orbit_online.cuh:186-231) into 9 SoA arrays */
@@ -292,6 +294,7 @@ struct VissArgs {
   int64_t npix;
   int64_t tile0, ntile; /* tiles of this pass: [tile0, tile0 + ntile) */
   const RowRec* rows;
+  const double4* rowt; /* (u, v, w) * 1024 per half row (plain form) */
   const double* ant8;
   const uint32_t* cls; /* [ntile][nh_pad] bytes (pass-relative) viewed as words */
   int64_t nh, nh_pad;
@@ -305,33 +308,52 @@ __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
   *c = __builtin_amdgcn_cosf(r);
 }
 
-/* one term: accumulate bb * exp(-j 2 pi x) (sign of the imaginary part is
- * applied when the partial is written) */
-template <bool MIXED>
-__device__ __forceinline__ void vw_viss_term(const RowRec& rec, double l, double m, double nm1, double bd, float bf,
-                                             const double* tab, float& fre, float& fim, double& dre,
-                                             double& dim) {
+/* mixed form, one term: accumulate bf * exp(-j 2 pi x) in FP32 (the sign of
+ * the imaginary part is applied when the partial is written) */
+__device__ __forceinline__ void vw_viss_term_mixed(const RowRec& rec, double l, double m, double nm1, float bf,
+                                                   float& fre, float& fim) {
   const double x = vw_viss_phase(rec.u, rec.v, rec.w, l, m, nm1);
-  if (MIXED) {
-    const float f = (float)(x - rint(x));
-    float s, c;
-    vw_hw_sincos_turn(f, &s, &c);
-    fre = fmaf(bf, c, fre);
-    fim = fmaf(bf, s, fim);
-  } else {
-    double s, c;
-    vw_sincos_turn_tab(x, tab, &s, &c);
-    dre = fma(bd, c, dre);
-    dim = fma(bd, s, dim);
+  const float f = (float)(x - rint(x));
+  float s, c;
+  vw_hw_sincos_turn(f, &s, &c);
+  fre = fmaf(bf, c, fre);
+  fim = fmaf(bf, s, fim);
+}
+
+/* plain form, the PPL terms of one lane for one half row: phases (operands
+ * pre-scaled by 1024), all table indices, the table reads, then polynomials
+ * and rotations, so the LDS reads overlap the arithmetic */
+template <int PPL>
+__device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const double* pl, const double* pm,
+                                                    const double* pz, const double* bw, const double2* tab,
+                                                    double& dre, double& dim) {
+  double f[PPL];
+  int ix[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) f[k] = vw_tab_split(vw_viss_phase(rt.x, rt.y, rt.z, pl[k], pm[k], pz[k]), &ix[k]);
+  double2 e[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    double sy, cy, sn, cs;
+    vw_tab_poly(f[k], &sy, &cy);
+    vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
+    dre = fma(bw[k], cs, dre);
+    dim = fma(bw[k], sn, dim);
   }
 }
 
 template <int TP, int RPW, bool MIXED, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_viss(VissArgs a) {
-  constexpr int PPL = TP / 64;
-  __shared__ double tab[2 * VW_TAB_N];
+  /* a tile is walked in sub-tiles of 256 pixels (4 per lane), so the
+   * register footprint does not grow with the tile size */
+  constexpr int PPL = 4;
+  constexpr int NSUB = TP / 256;
+  static_assert(RPW % 4 == 0, "class words hold 4 rows");
+  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
   if (!MIXED) {
-    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, tab);
+    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, (double*)tab);
     __syncthreads();
   }
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
@@ -339,7 +361,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int64_t rb = (vb - chunk * a.bpc) * 4 + wave;
   if (chunk >= a.nchunk || rb >= a.nrb) return;
-  const int64_t r0 = rb * RPW;
+  const int64_t r0 = rb * RPW; /* r0 + RPW <= nh_pad: nh_pad is a multiple of 64 */
   const int64_t t0 = a.tile0 + chunk * a.tpc;
   const int64_t tend = a.tile0 + a.ntile;
   const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
@@ -350,13 +372,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
   for (int r = 0; r < RPW; ++r) { fre[r] = 0.f; fim[r] = 0.f; dre[r] = 0.0; dim[r] = 0.0; cnt[r] = 0; }
 
-  for (int64_t tile = t0; tile < t1; ++tile) {
+  for (int64_t st = t0 * NSUB; st < t1 * NSUB; ++st) {
+    const int64_t tile = st / NSUB;
+    /* the class bytes of this wave's RPW rows for the tile: RPW/4 uniform words */
+    uint32_t cw[RPW / 4];
+    const uint32_t* cwp = a.cls + (((tile - a.tile0) * a.nh_pad + r0) >> 2);
+#pragma unroll
+    for (int j = 0; j < RPW / 4; ++j) cw[j] = (uint32_t)vw_rfl((int)cwp[j]);
     double pl[PPL], pm[PPL], pn[PPL], pz[PPL], pb[PPL];
     float pbf[PPL];
     bool ok[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      int64_t p = tile * TP + lane + 64 * k;
+      int64_t p = st * 256 + lane + 64 * k;
       ok[k] = p < a.npix;
       int64_t pp = ok[k] ? p : 0;
       pl[k] = a.l[pp];
@@ -370,30 +398,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int64_t h = r0 + r;
-      if (h < a.nh) {
-        const int cls = vw_cls_lookup(a.cls, tile - a.tile0, a.nh_pad, h);
-        if (cls == VW_CLS_VISIBLE) {
+      /* rows past nh carry the hidden class */
+      const int cls = (int)((cw[r >> 2] >> (8 * (r & 3))) & 0xffu);
+      if (cls == VW_CLS_VISIBLE) {
+        if (MIXED) {
           const RowRec rec = a.rows[h];
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) {
-            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], MIXED ? 0.0 : pb[k], MIXED ? pbf[k] : 0.f, tab, fre[r],
-                                fim[r], dre[r], dim[r]);
-            if (COUNT) cnt[r] += ok[k] ? 1 : 0;
-          }
-        } else if (cls == VW_CLS_MIXED) {
-          const RowRec rec = a.rows[h];
-          const double* qa = a.ant8 + (int64_t)rec.ia * 8;
-          const double* qb = a.ant8 + (int64_t)rec.ib * 8;
-          const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
-          const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
+          for (int k = 0; k < PPL; ++k) vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], pbf[k], fre[r], fim[r]);
+        } else {
+          vw_viss_terms_plain<PPL>(a.rowt[h], pl, pm, pz, pb, tab, dre[r], dim[r]);
+        }
+        if (COUNT) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) {
-            const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
-            vw_viss_term<MIXED>(rec, pl[k], pm[k], pz[k], vis ? pb[k] : 0.0, vis ? pbf[k] : 0.f, tab, fre[r],
-                                fim[r], dre[r], dim[r]);
-            if (COUNT) cnt[r] += vis ? 1 : 0;
-          }
+          for (int k = 0; k < PPL; ++k) cnt[r] += ok[k] ? 1 : 0;
+        }
+      } else if (cls == VW_CLS_MIXED) {
+        const RowRec rec = a.rows[h];
+        const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+        const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+        const double ax = qa[0], ay = qa[1], az = qa[2], ai = qa[3], ac = qa[7];
+        const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
+        bool vis[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+          vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                   vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+        if (MIXED) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k)
+            vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], vis[k] ? pbf[k] : 0.f, fre[r], fim[r]);
+        } else {
+          double bw[PPL];
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) bw[k] = vis[k] ? pb[k] : 0.0;
+          vw_viss_terms_plain<PPL>(a.rowt[h], pl, pm, pz, bw, tab, dre[r], dim[r]);
+        }
+        if (COUNT) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) cnt[r] += vis[k] ? 1 : 0;
         }
       }
     }
@@ -622,7 +664,7 @@ struct GRec {
   double ugu, vgu, vr, vi;
 };
 struct GroupSoA {
-  GRec* rec;    /* cell centre (wavelengths) and visibility, 32 B      */
+  GRec* rec;    /* cell centre (wavelengths; x1024 in the plain form) and visibility, 32 B */
   double* ant;  /* [q][10]: representative's antenna frames a and b    */
   int32_t *gia, *gib, *grep;
 };
@@ -630,7 +672,7 @@ struct GroupSoA {
 __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* __restrict__ goff,
                              const int32_t* __restrict__ idx_sorted, const double2* __restrict__ vh,
                              const RowRec* __restrict__ rows, const double* __restrict__ ant8, int P, int full,
-                             int blockage,
+                             int blockage, double uv_scale,
                              const DevInfo* __restrict__ info, GroupSoA g) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= info->nuniq) return;
@@ -661,8 +703,8 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
     vi = si / (double)(s1 - s0);
   }
   GRec gr;
-  gr.ugu = ugu;
-  gr.vgu = vgu;
+  gr.ugu = ugu * uv_scale; /* 1024 for the plain table sincos, 1 for the mixed form (exact) */
+  gr.vgu = vgu * uv_scale;
   gr.vr = vr;
   gr.vi = vi;
   g.rec[q] = gr;
@@ -739,27 +781,11 @@ __device__ __forceinline__ void vw_ant_vis(const double* __restrict__ ant8, int 
              vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
 }
 
-template <bool MIXED>
-struct ReconAcc;
-
-template <>
-struct ReconAcc<false> {
-  double a = 0.0;
-  __device__ void term(double ugu, double vgu, double vr, double vi, float, float, double lp, double mp,
-                       const double* tab) {
-    a = a + vw_recon_term(vr, vi, vw_recon_phase(ugu, vgu, lp, mp), tab);
-  }
-  __device__ void fold() {}
-  __device__ double value() const { return a; }
-};
-
 /* FP64 phase and turn reduction, hardware FP32 sin/cos, FP32 block sums
- * folded with Kahan compensation */
-template <>
-struct ReconAcc<true> {
+ * folded with Kahan compensation (mixed form, one point) */
+struct ReconAccF32 {
   float s = 0.f, c = 0.f, b = 0.f;
-  __device__ void term(double ugu, double vgu, double, double, float vrf, float vif, double lp, double mp,
-                       const double*) {
+  __device__ void term(double ugu, double vgu, float vrf, float vif, double lp, double mp) {
     const double x = vw_recon_phase(ugu, vgu, lp, mp);
     const float f = (float)(x - rint(x));
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
@@ -775,11 +801,57 @@ struct ReconAcc<true> {
   __device__ double value() const { return ((double)s - (double)c) + (double)b; }
 };
 
+/* the PPL points of one lane and one group.  Plain: (ugu, vgu) arrive
+ * scaled by 1024; all table indices and reads come before the polynomials
+ * so the LDS reads overlap the arithmetic; a point with use[k] false keeps
+ * its sum. */
+template <int PPL, bool MIXED>
+struct ReconLane {
+  double d[PPL];
+  ReconAccF32 f[PPL];
+  __device__ ReconLane() {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) d[k] = 0.0;
+  }
+  __device__ __forceinline__ void add(double ugu, double vgu, const double* vr, const double* vi, const bool* use,
+                                      const double* lp, const double* mp, const double2* tab) {
+    if (MIXED) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k)
+        if (use[k]) f[k].term(ugu, vgu, (float)vr[k], (float)vi[k], lp[k], mp[k]);
+    } else {
+      double fr[PPL];
+      int ix[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) fr[k] = vw_tab_split(vw_recon_phase(ugu, vgu, lp[k], mp[k]), &ix[k]);
+      double2 e[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        double sy, cy, sn, cs;
+        vw_tab_poly(fr[k], &sy, &cy);
+        vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
+        const double t = vw_recon_add(d[k], vr[k], vi[k], sn, cs);
+        d[k] = use[k] ? t : d[k];
+      }
+    }
+  }
+  __device__ void fold() {
+    if (MIXED) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) f[k].fold();
+    }
+  }
+  __device__ double value(int k) const { return MIXED ? f[k].value() : d[k]; }
+};
+
 template <int PPL, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
-  __shared__ double tab[2 * VW_TAB_N];
+  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
+  __shared__ double gring[4][2 * 64]; /* per wave: two blocks of 16 group records */
   if (!MIXED) {
-    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, tab);
+    for (int k = threadIdx.x; k < VW_TAB_N; k += 256) vw_sincos_tab_build(k, (double*)tab);
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
@@ -792,10 +864,10 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   const vw_segparams sp = a.info->sp;
   const int64_t nu = a.info->nuniq;
   const int64_t q0 = a.info->first_nz;
-  /* group range of this chunk, aligned to 4 groups */
-  const int64_t base = q0 & ~(int64_t)3;
+  /* group range of this chunk, aligned to 16 groups */
+  const int64_t base = q0 & ~(int64_t)15;
   const int64_t span = ((nu - base) + a.nchunk - 1) / a.nchunk;
-  const int64_t spanA = (span + 3) & ~(int64_t)3;
+  const int64_t spanA = (span + 15) & ~(int64_t)15;
   const int64_t qa_ = base + chunk * spanA;
   const int64_t qb_ = qa_ + spanA < nu ? qa_ + spanA : nu;
 
@@ -812,7 +884,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
     lp[k] = fma(sp.fa, pn[k], pl[k]);
     mp[k] = fma(sp.fb, pn[k], pm[k]);
   }
-  ReconAcc<MIXED> acc[PPL];
+  ReconLane<PPL, MIXED> acc;
   uint32_t wl[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) wl[k] = 0;
@@ -820,54 +892,61 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   int nblk = 0;
 
   if (KIND == VW_KIND_REP || KIND == VW_KIND_AVG) {
-    /* blocks of 4 groups; the next block's records and class word are
-     * fetched before the current block is evaluated */
-    GRec nx[4];
+    /* records pass through a per-wave LDS ring in blocks of 16 (512 B: one
+     * double per lane, loaded one block ahead); each group is then read
+     * with a uniform LDS address.  Class words (4 groups each) are scalar
+     * loads fetched one word ahead.  Blocks start at multiples of 16 and
+     * nq_pad is a multiple of 64, so every block lies inside the records. */
+    double* ring = gring[vw_rfl((int)(threadIdx.x >> 6))];
+    const double* recd = (const double*)a.g.rec;
+    double stage = (qa_ < qb_) ? recd[qa_ * 4 + lane] : 0.0;
     uint32_t ncw = 0x01010101u; /* AVG: every group counts, no class */
-    if (qa_ < qb_) {
+    if (KIND == VW_KIND_REP && qa_ < qb_) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + qa_) >> 2]);
+    int buf = 0;
+    for (int64_t q16 = qa_; q16 < qb_; q16 += 16, buf ^= 1) {
+      ring[buf * 64 + lane] = stage;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (q16 + 16 < qb_) stage = recd[(q16 + 16) * 4 + lane];
+      const GRec* blk = (const GRec*)(ring + buf * 64);
+      const int64_t qe = q16 + 16 < qb_ ? q16 + 16 : qb_;
+      for (int64_t q4 = q16; q4 < qe; q4 += 4) {
+        const uint32_t cw = ncw;
+        if (KIND == VW_KIND_REP && q4 + 4 < qb_)
+          ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + q4 + 4) >> 2]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) nx[j] = a.g.rec[qa_ + j];
-      if (KIND == VW_KIND_REP) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + qa_) >> 2]);
-    }
-    for (int64_t q4 = qa_; q4 < qb_; q4 += 4) {
-      GRec cur[4];
+        for (int j = 0; j < 4; ++j) {
+          const int64_t q = q4 + j;
+          const int cls = (int)((cw >> (8 * j)) & 0xffu);
+          if (q < q0 || q >= qb_ || cls == VW_CLS_HIDDEN) continue;
+          const GRec g = blk[q - q16];
+          double vr[PPL], vi[PPL];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) cur[j] = nx[j];
-      const uint32_t cw = ncw;
-      if (q4 + 4 < qb_) {
+          for (int k = 0; k < PPL; ++k) { vr[k] = g.vr; vi[k] = g.vi; }
+          if (cls == VW_CLS_VISIBLE) {
+            bool all[PPL];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nx[j] = a.g.rec[q4 + 4 + j];
-        if (KIND == VW_KIND_REP) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + q4 + 4) >> 2]);
-      }
+            for (int k = 0; k < PPL; ++k) all[k] = true;
+            acc.add(g.ugu, g.vgu, vr, vi, all, lp, mp, tab);
+            wu += 1;
+          } else {
+            const double* d = a.g.ant + q * 10;
+            const double ax = d[0], ay = d[1], az = d[2], ai = d[3], ac = d[4];
+            const double bx = d[5], by = d[6], bz = d[7], bi = d[8], bc = d[9];
+            bool vis[PPL];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t q = q4 + j;
-        const int cls = (int)((cw >> (8 * j)) & 0xffu);
-        if (q < q0 || q >= qb_ || cls == VW_CLS_HIDDEN) continue;
-        const GRec g = cur[j];
-        const float vrf = (float)g.vr, vif = (float)g.vi;
-        if (cls == VW_CLS_VISIBLE) {
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k], tab);
-          wu += 1;
-        } else {
-          const double* d = a.g.ant + q * 10;
-          const double ax = d[0], ay = d[1], az = d[2], ai = d[3], ac = d[4];
-          const double bx = d[5], by = d[6], bz = d[7], bi = d[8], bc = d[9];
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) {
-            const bool vis = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
-            if (vis) {
-              acc[k].term(g.ugu, g.vgu, g.vr, g.vi, vrf, vif, lp[k], mp[k], tab);
-              wl[k] += 1;
+            for (int k = 0; k < PPL; ++k) {
+              vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                       vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+              wl[k] += vis[k] ? 1 : 0;
             }
+            acc.add(g.ugu, g.vgu, vr, vi, vis, lp, mp, tab);
           }
-        }
-        if (MIXED && ++nblk == 32) {
-          nblk = 0;
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) acc[k].fold();
+          if (MIXED && ++nblk == 32) {
+            nblk = 0;
+            acc.fold();
+          }
         }
       }
     }
@@ -908,19 +987,20 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
           if (mv[k] && fin) { sr[k] = sr[k] + wr; si[k] = si[k] + wi; cn[k] += 1; }
       }
       const double ugu = a.g.rec[q].ugu, vgu = a.g.rec[q].vgu;
+      double ar[PPL], ai[PPL];
+      bool use[PPL];
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        if (cn[k] > 0) {
-          const double invc = 1.0 / (double)cn[k];
-          const double ar = sr[k] * invc, ai = si[k] * invc;
-          acc[k].term(ugu, vgu, ar, ai, (float)ar, (float)ai, lp[k], mp[k], tab);
-          wl[k] += 1;
-        }
+        use[k] = cn[k] > 0;
+        const double invc = 1.0 / (double)(use[k] ? cn[k] : 1);
+        ar[k] = sr[k] * invc;
+        ai[k] = si[k] * invc;
+        wl[k] += use[k] ? 1 : 0;
       }
+      acc.add(ugu, vgu, ar, ai, use, lp, mp, tab);
       if (MIXED && ++nblk == 32) {
         nblk = 0;
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) acc[k].fold();
+        acc.fold();
       }
     }
   }
@@ -928,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
   for (int k = 0; k < PPL; ++k) {
     if (!ok[k]) continue;
     const int64_t p = pbase + lane + 64 * k;
-    a.part[chunk * a.pstride + p] = acc[k].value();
+    a.part[chunk * a.pstride + p] = acc.value(k);
     a.wpart[chunk * a.pstride + p] = wu + wl[k];
   }
 }

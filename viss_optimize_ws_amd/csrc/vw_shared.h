@@ -94,32 +94,62 @@ VW_HD void vw_sincos_turn(double r, double* s_out, double* c_out) {
   *c_out = ((qi + 1) & 2) ? -b : b;
 }
 
-/* Table-driven sin/cos of 2*pi*x (Tang's method): with r = x - rint(x),
- * r*256 = k + f exactly with |f| <= 1/2, so 2*pi*x = 2*pi*k/256 + y with |y| <= pi/256; sin/cos(y) by
- * short Taylor polynomials (omitted terms < 2e-20), then one rotation by the
- * table entry (cos, sin)(2*pi*k/256).  The table is built by
- * vw_sincos_tab_build with vw_sincos_turn, identically on host and device.
- * Error within ~2 ulp of sin/cos; about 20 operations plus one table read. */
-#define VW_TAB_N 256
+/* Table-driven sin/cos of 2*pi*x (Tang's method).  The argument is passed
+ * as t = 1024 x (x in turns; the kernels fold the factor 1024 = 2^10 into
+ * their operands, which is exact).  t = k + f with k = rint(t) and
+ * |f| <= 1/2 exactly, so 2*pi*x = 2*pi*k/1024 + y with |y| <= pi/1024;
+ * sin(y) = y - y^3/6 + y^5/120 (omitted term < 6e-22) and
+ * cos(y) = 1 - y^2/2 + y^4/24 (omitted term < 1.2e-18), then one rotation
+ * by the table entry (cos, sin)(2*pi*(k mod 1024)/1024).  The table is built
+ * by vw_sincos_tab_build with vw_sincos_turn, identically on host and
+ * device.  Error within ~2 ulp of sin/cos; 15 operations plus one table
+ * read. */
+#define VW_TAB_N 1024
+#define VW_TAB_SCALE 1024.0
 VW_HD void vw_sincos_tab_build(int k, double* tab) {
   vw_sincos_turn((double)k / (double)VW_TAB_N, &tab[2 * k + 1], &tab[2 * k]);
 }
 
-VW_HD void vw_sincos_turn_tab(double x, const double* tab, double* s_out, double* c_out) {
-  double r = x - rint(x);           /* exact, |r| <= 1/2 turn */
-  double t = r * 256.0;             /* exact */
-  double k = rint(t);               /* -128 .. 128 */
-  double f = t - k;                 /* exact, |f| <= 1/2 */
-  double y = fma(f, 0x1.921fb54442d18p-6, f * 0x1.1a62633145c07p-60); /* f * 2 pi / 256 */
+/* low 32 bits of a double's encoding */
+VW_HD uint32_t vw_lo32(double d) {
+  union { double d; uint64_t u; } v;
+  v.d = d;
+  return (uint32_t)v.u;
+}
+
+/* t -> f = t - rint(t) (exact) and the table index rint(t) mod 1024.
+ * Adding 1.5*2^52 rounds t to an integer k (ties to even, as rint) held in
+ * the low mantissa bits as 2^51 + k; valid for |t| < 2^51 (|x| < 2^41 turns). */
+VW_HD double vw_tab_split(double t, int* idx) {
+  const double M = 0x1.8p52;
+  double sh = t + M;
+  double k = sh - M;
+  *idx = (int)(vw_lo32(sh) & (VW_TAB_N - 1));
+  return t - k;
+}
+
+/* sin and cos of y = 2*pi*f/1024, |f| <= 1/2 */
+VW_HD void vw_tab_poly(double f, double* sy, double* cy) {
+  double y = f * 0x1.921fb54442d18p-8; /* 2*pi/1024 */
   double z = y * y;
-  double ps = fma(fma(-0x1.a01a01a01a01ap-13, z, 0x1.1111111111111p-7), z, -0x1.5555555555555p-3);
-  double sy = fma(y * z, ps, y);
-  double pc = fma(fma(-0x1.6c16c16c16c17p-10, z, 0x1.5555555555555p-5), z, -0.5);
-  double cy = fma(z, pc, 1.0);
-  int ki = ((int)k) & (VW_TAB_N - 1);
-  double tc = tab[2 * ki], ts = tab[2 * ki + 1];
-  *s_out = fma(ts, cy, tc * sy);
-  *c_out = fma(tc, cy, -(ts * sy));
+  double ps = fma(z, 0x1.1111111111111p-7, -0x1.5555555555555p-3); /* 1/120, -1/6 */
+  *sy = fma(y * z, ps, y);
+  *cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0); /* 1/24 */
+}
+
+/* rotation by the table entry (ec, es) */
+VW_HD void vw_tab_rotate(double ec, double es, double sy, double cy, double* s_out, double* c_out) {
+  *s_out = fma(es, cy, ec * sy);
+  *c_out = fma(ec, cy, -(es * sy));
+}
+
+/* sin and cos of 2*pi*t/1024 */
+VW_HD void vw_sincos_tab(double t, const double* tab, double* s_out, double* c_out) {
+  int i;
+  double f = vw_tab_split(t, &i);
+  double sy, cy;
+  vw_tab_poly(f, &sy, &cy);
+  vw_tab_rotate(tab[2 * i], tab[2 * i + 1], sy, cy, s_out, c_out);
 }
 
 VW_HD double vw_dot3(double ax, double ay, double az, double bx, double by, double bz) {
@@ -480,11 +510,10 @@ VW_HD double vw_recon_phase(double ugu, double vgu, double lp, double mp) {
   return fma(vgu, mp, ugu * lp);
 }
 
-/* Re(V exp(+j 2 pi x)) = Vr cos - Vi sin */
-VW_HD double vw_recon_term(double vr, double vi, double x, const double* tab) {
-  double s, c;
-  vw_sincos_turn_tab(x, tab, &s, &c);
-  return fma(vr, c, -(vi * s));
+/* acc + Re(V exp(+j 2 pi x)) = acc + Vr cos - Vi sin, the phase passed as
+ * t = 1024 x; the sine product is added first */
+VW_HD double vw_recon_add(double acc, double vr, double vi, double s, double c) {
+  return fma(vr, c, fma(-vi, s, acc));
 }
 
 /* lm-grid image point (row i -> m, column j -> l), pixel centres on [-1,1]:
