@@ -196,6 +196,17 @@ int vw_finalize(void* ctx, double* C_healpix, uint32_t* W_healpix, double* img,
  * mirror_tile_vissfast/main_recon_online_mirror_tile_vissfast.cu:257-265). */
 int vw_reset_accumulators(void* ctx);
 
+/* Raw accumulators of the current day, before normalisation: per-pixel sums
+ * and contributing-key counts of the HEALPix map (npix each) and of the
+ * lm-grid points (n_img_pts each, vw_get_img_points order), and the running
+ * key total that blockage=0 divides by (scale_inplace).  For segment-level
+ * checkpoint / resume: the reference keeps Cacc/Wacc only in device memory
+ * between segments (add_inplace, viss_recon_kernel.cuh:641-659).
+ * get: NULL pointers are skipped.  set: a NULL array is cleared. */
+int vw_get_accumulators(void* ctx, double* C, uint32_t* W, double* G, uint32_t* GW, int64_t* wtotal);
+int vw_set_accumulators(void* ctx, const double* C, const uint32_t* W, const double* G, const uint32_t* GW,
+                        int64_t wtotal);
+
 int vw_get_timings(void* ctx, vw_timings* out);
 
 /* ---------------- introspection (tests) ---------------- */

@@ -145,6 +145,8 @@ DEVICE_ONLY = {
     "run_all": (C.c_int, [P]),
     "set_day": (C.c_int, [P, i32]),
     "get_timings": (C.c_int, [P, C.POINTER(vw_timings)]),
+    "get_accumulators": (C.c_int, [P, PD, PU32, PD, PU32, PI64]),
+    "set_accumulators": (C.c_int, [P, PD, PU32, PD, PU32, i64]),
     "get_viss_counts": (C.c_int, [P, PI64]),
     "make_sky": (C.c_int, [C.c_uint64, i64, i32, PD]),
     "io_load_sky": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, i64, PD, C.c_char_p, i32]),

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -942,6 +943,38 @@ VW_API int vw_reset_accumulators(void* p) {
     HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
+  return sync(c);
+}
+
+VW_API int vw_get_accumulators(void* p, double* C, uint32_t* W, double* G, uint32_t* GW, int64_t* wtotal) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (C) HC(hipMemcpyAsync(C, c->d_Cacc, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
+  if (W) HC(hipMemcpyAsync(W, c->d_Wacc, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
+  if (c->ngp && G) HC(hipMemcpyAsync(G, c->d_Gacc, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->st));
+  if (c->ngp && GW) HC(hipMemcpyAsync(GW, c->d_GWacc, sizeof(uint32_t) * c->ngp, hipMemcpyDeviceToHost, c->st));
+  if (wtotal)
+    HC(hipMemcpyAsync(wtotal, (char*)c->d_info + offsetof(DevInfo, wtotal), sizeof(int64_t), hipMemcpyDeviceToHost,
+                      c->st));
+  return sync(c);
+}
+
+VW_API int vw_set_accumulators(void* p, const double* C, const uint32_t* W, const double* G, const uint32_t* GW,
+                               int64_t wtotal) {
+  Ctx* c = as_ctx(p);
+  if (!c || wtotal < 0) return VW_ERR_ARG;
+  if (C) HC(hipMemcpyAsync(c->d_Cacc, C, sizeof(double) * c->npix, hipMemcpyHostToDevice, c->st));
+  else HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
+  if (W) HC(hipMemcpyAsync(c->d_Wacc, W, sizeof(uint32_t) * c->npix, hipMemcpyHostToDevice, c->st));
+  else HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * c->npix, c->st));
+  if (c->ngp) {
+    if (G) HC(hipMemcpyAsync(c->d_Gacc, G, sizeof(double) * c->ngp, hipMemcpyHostToDevice, c->st));
+    else HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    if (GW) HC(hipMemcpyAsync(c->d_GWacc, GW, sizeof(uint32_t) * c->ngp, hipMemcpyHostToDevice, c->st));
+    else HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
+  }
+  HC(hipMemcpyAsync((char*)c->d_info + offsetof(DevInfo, wtotal), &wtotal, sizeof(int64_t), hipMemcpyHostToDevice,
+                    c->st));
   return sync(c);
 }
 

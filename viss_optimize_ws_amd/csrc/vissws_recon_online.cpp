@@ -15,6 +15,11 @@
  *   --precision={plain,mixed} --recon={full,mirror} --img_n --bl_max_m
  *   --sky={file,lognormal,lognormal_ps} --sky_seed --occ_radius_km
  *   --occ_altitude_km
+ *   --checkpoint=PATH  segment-level checkpoint (raw accumulators, position
+ *                      in the schedule, text-output sizes) after every
+ *                      segment and day; --resume=1 continues from it;
+ *                      --stop_after_segs=N ends this invocation after N
+ *                      segments (the checkpoint is kept)
  * Outputs (out_dir): C{day}day{btag}.{txt|bin} (npix float32, NESTED),
  * Viss{day}day{btag}.txt, uvw/xyza/xyzb{day}day{btag}.txt, and with
  * --img_n>0 img{day}day{btag}.bin (img_n^2 float64, row = m, col = l).
@@ -26,6 +31,7 @@
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 
 #include "../../include/vissws.h"
@@ -71,6 +77,70 @@ int die(void* ctx, const char* what) {
   return 1;
 }
 
+/* ---- segment-level checkpoint ----
+ * Layout: Ckpt header, then (when next_seg > 0) C[npix] f64, W[npix] u32,
+ * G[ngp] f64, GW[ngp] u32.  Written to PATH.tmp and renamed over PATH, so an
+ * interrupted write leaves the previous checkpoint. */
+struct Ckpt {
+  char magic[8];
+  uint64_t fingerprint; /* run parameters that must match on resume */
+  int32_t day, next_seg; /* next work item: segment next_seg of day      */
+  int64_t npix, ngp, wtotal;
+  int64_t fsize[4]; /* Viss, uvw, xyza, xyzb text sizes (-1: absent)     */
+};
+const char kMagic[8] = {'V', 'W', 'C', 'K', 'P', 'T', '0', '1'};
+
+struct Fnv {
+  uint64_t h = 1469598103934665603ULL;
+  void add(const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ULL;
+  }
+  template <class T> void v(T x) { add(&x, sizeof x); }
+  void str(const std::string& s) { add(s.data(), s.size()); v((uint64_t)s.size()); }
+};
+
+int64_t file_size(const std::string& f) {
+  struct stat st;
+  return stat(f.c_str(), &st) == 0 ? (int64_t)st.st_size : -1;
+}
+
+bool write_ckpt(const std::string& path, const Ckpt& h, const std::vector<double>& C, const std::vector<uint32_t>& W,
+                const std::vector<double>& G, const std::vector<uint32_t>& GW) {
+  std::string tmp = path + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return false;
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  if (ok && h.next_seg > 0)
+    ok = std::fwrite(C.data(), sizeof(double), (size_t)h.npix, f) == (size_t)h.npix &&
+         std::fwrite(W.data(), sizeof(uint32_t), (size_t)h.npix, f) == (size_t)h.npix &&
+         std::fwrite(G.data(), sizeof(double), (size_t)h.ngp, f) == (size_t)h.ngp &&
+         std::fwrite(GW.data(), sizeof(uint32_t), (size_t)h.ngp, f) == (size_t)h.ngp;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok && std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+/* 1 = loaded, 0 = absent, -1 = unreadable */
+int read_ckpt(const std::string& path, Ckpt& h, std::vector<double>& C, std::vector<uint32_t>& W,
+              std::vector<double>& G, std::vector<uint32_t>& GW) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return 0;
+  bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, kMagic, 8) == 0 && h.npix >= 0 &&
+            h.ngp >= 0;
+  if (ok && h.next_seg > 0) {
+    C.resize((size_t)h.npix);
+    W.resize((size_t)h.npix);
+    G.resize((size_t)h.ngp);
+    GW.resize((size_t)h.ngp);
+    ok = std::fread(C.data(), sizeof(double), (size_t)h.npix, f) == (size_t)h.npix &&
+         std::fread(W.data(), sizeof(uint32_t), (size_t)h.npix, f) == (size_t)h.npix &&
+         std::fread(G.data(), sizeof(double), (size_t)h.ngp, f) == (size_t)h.ngp &&
+         std::fread(GW.data(), sizeof(uint32_t), (size_t)h.ngp, f) == (size_t)h.ngp;
+  }
+  std::fclose(f);
+  return ok ? 1 : -1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -106,6 +176,9 @@ int main(int argc, char** argv) {
   cfg.bl_max_m = to_dbl(get_arg(argc, argv, "--bl_max_m", "0"), 0.0);
   std::string sky_kind = get_arg(argc, argv, "--sky", "file");
   uint64_t sky_seed = to_u64(get_arg(argc, argv, "--sky_seed", "1"), 1);
+  std::string ckpt_path = get_arg(argc, argv, "--checkpoint", "");
+  int resume = to_int(get_arg(argc, argv, "--resume", "0"), 0);
+  int stop_after = to_int(get_arg(argc, argv, "--stop_after_segs", "0"), 0);
   if (geometry == VW_GEOM_CIRCULAR) {
     cfg.n_ant = to_int(get_arg(argc, argv, "--n_ant", "8"), 8);
     cfg.n_epochs = to_int(get_arg(argc, argv, "--n_epochs", "64"), 64);
@@ -197,6 +270,80 @@ int main(int argc, char** argv) {
 
   std::vector<int32_t> seg_t0(plan.segs), seg_tlen(plan.segs);
   vw_get_schedule(ctx, seg_t0.data(), seg_tlen.data());
+
+  /* checkpoint identity: everything that changes the accumulated values */
+  Fnv fp;
+  fp.v(cfg.nside); fp.v(cfg.freq_hz); fp.v((int32_t)geometry); fp.v(cfg.n_ant); fp.v(plan.n_epochs);
+  fp.v(cfg.dt_s); fp.v(cfg.orbit_seed); fp.v(plan.segs); fp.v(cfg.blockage); fp.v(cfg.occ_mode);
+  fp.v(cfg.recon_mode); fp.v(cfg.precision); fp.v(cfg.tile_pix); fp.v(cfg.img_n); fp.v(plan.bl_max_m);
+  fp.v(cfg.occ_radius_km); fp.v(cfg.occ_altitude_km); fp.v(cfg.r_min_km); fp.v(cfg.r_max_km);
+  fp.str(btag); fp.str(sky_kind); fp.v(sky_seed); fp.str(sky_kind == "file" ? sky_dir : std::string());
+  fp.v(write_viss); fp.v(write_baseline_txt);
+  for (int k = 0; k < plan.segs; ++k) { fp.v(seg_t0[k]); fp.v(seg_tlen[k]); }
+  std::vector<double> ckC, ckG;
+  std::vector<uint32_t> ckW, ckGW;
+  int d_begin = 0, s_begin = 0;
+  int64_t ck_wtotal = 0;
+  bool restore = false;
+  if (resume && !ckpt_path.empty()) {
+    Ckpt h;
+    int r = read_ckpt(ckpt_path, h, ckC, ckW, ckG, ckGW);
+    if (r < 0) {
+      std::fprintf(stderr, "ERROR unreadable checkpoint %s\n", ckpt_path.c_str());
+      return 1;
+    }
+    if (r > 0) {
+      if (h.fingerprint != fp.h || h.npix != npix || h.ngp != plan.n_img_pts) {
+        std::fprintf(stderr, "ERROR checkpoint %s was written by a run with other parameters\n", ckpt_path.c_str());
+        return 1;
+      }
+      if (h.day < day_start || h.day > day_start + day_count || h.next_seg < 0 || h.next_seg > plan.segs) {
+        std::fprintf(stderr, "ERROR checkpoint %s is outside days [%d, %d]\n", ckpt_path.c_str(), day_start,
+                     day_start + day_count - 1);
+        return 1;
+      }
+      d_begin = h.day - day_start;
+      s_begin = h.next_seg;
+      restore = s_begin > 0;
+      if (restore) {
+        /* drop text rows written after the checkpoint */
+        std::string tagday = std::to_string(h.day) + "day" + btag;
+        const std::string names[4] = {out_dir + "Viss" + tagday + ".txt", out_dir + "uvw" + tagday + ".txt",
+                                      out_dir + "xyza" + tagday + ".txt", out_dir + "xyzb" + tagday + ".txt"};
+        for (int k = 0; k < 4; ++k)
+          if (h.fsize[k] >= 0 && truncate(names[k].c_str(), (off_t)h.fsize[k]) != 0) {
+            std::fprintf(stderr, "ERROR cannot restore %s to its checkpointed size\n", names[k].c_str());
+            return 1;
+          }
+        ck_wtotal = h.wtotal;
+      }
+      std::printf("Resuming from %s: day %d segment %d\n", ckpt_path.c_str(), h.day, s_begin + 1);
+    }
+  }
+  auto save_ckpt = [&](int dayid, int next_seg) -> bool {
+    if (ckpt_path.empty()) return true;
+    Ckpt h;
+    std::memcpy(h.magic, kMagic, 8);
+    h.fingerprint = fp.h;
+    h.day = dayid;
+    h.next_seg = next_seg;
+    h.npix = npix;
+    h.ngp = plan.n_img_pts;
+    h.wtotal = 0;
+    std::string tagday = std::to_string(dayid) + "day" + btag;
+    const std::string names[4] = {out_dir + "Viss" + tagday + ".txt", out_dir + "uvw" + tagday + ".txt",
+                                  out_dir + "xyza" + tagday + ".txt", out_dir + "xyzb" + tagday + ".txt"};
+    for (int k = 0; k < 4; ++k) h.fsize[k] = next_seg > 0 ? file_size(names[k]) : -1;
+    if (next_seg > 0) {
+      ckC.resize((size_t)npix);
+      ckW.resize((size_t)npix);
+      ckG.resize((size_t)plan.n_img_pts);
+      ckGW.resize((size_t)plan.n_img_pts);
+      if (vw_get_accumulators(ctx, ckC.data(), ckW.data(), ckG.data(), ckGW.data(), &h.wtotal) != VW_OK) return false;
+    }
+    return write_ckpt(ckpt_path, h, ckC, ckW, ckG, ckGW);
+  };
+  int segs_done = 0;
   const int64_t max_nf = 2LL * plan.max_tlen * plan.n_pairs;
   std::vector<double> gbuf(9 * (size_t)max_nf);
   std::vector<vw_c128> vfull((size_t)max_nf);
@@ -204,18 +351,24 @@ int main(int argc, char** argv) {
   std::vector<uint32_t> Wmap((size_t)npix);
   std::vector<double> img((size_t)cfg.img_n * cfg.img_n);
 
-  for (int d = 0; d < day_count; ++d) {
+  for (int d = d_begin; d < day_count; ++d) {
     int dayid = day_start + d;
     std::printf("===== Day %d / %d =====\n", dayid, day_start + day_count - 1);
     if (d > 0 && vw_set_day(ctx, dayid) != VW_OK) return die(ctx, "vw_set_day");
-    if (vw_reset_accumulators(ctx) != VW_OK) return die(ctx, "vw_reset_accumulators");
+    const bool resumed_day = restore && d == d_begin;
+    if (resumed_day) {
+      if (vw_set_accumulators(ctx, ckC.data(), ckW.data(), ckG.data(), ckGW.data(), ck_wtotal) != VW_OK)
+        return die(ctx, "vw_set_accumulators");
+    } else if (vw_reset_accumulators(ctx) != VW_OK) {
+      return die(ctx, "vw_reset_accumulators");
+    }
     std::string tagday = std::to_string(dayid) + "day" + btag;
     std::string fviss = out_dir + "Viss" + tagday + ".txt";
     std::string fuvw = out_dir + "uvw" + tagday + ".txt", fxa = out_dir + "xyza" + tagday + ".txt",
                 fxb = out_dir + "xyzb" + tagday + ".txt";
     Timer t_rec;
     t_rec.tic();
-    for (int s = 0; s < plan.segs; ++s) {
+    for (int s = (d == d_begin ? s_begin : 0); s < plan.segs; ++s) {
       int t0 = seg_t0[s], tlen = seg_tlen[s];
       int64_t segN = 2LL * tlen * plan.n_pairs, segN_half = (int64_t)tlen * plan.n_pairs;
       bool need_geom = write_baseline_txt || print_stats;
@@ -269,6 +422,17 @@ int main(int argc, char** argv) {
       }
       std::printf("\n");
       std::fflush(stdout);
+      ++segs_done;
+      if (s + 1 < plan.segs && !save_ckpt(dayid, s + 1)) {
+        std::fprintf(stderr, "ERROR writing checkpoint %s\n", ckpt_path.c_str());
+        return 1;
+      }
+      if (stop_after > 0 && segs_done >= stop_after && s + 1 < plan.segs) {
+        std::printf("Stopped after %d segments%s%s\n", segs_done, ckpt_path.empty() ? "" : "; checkpoint ",
+                    ckpt_path.c_str());
+        vw_destroy(ctx);
+        return 0;
+      }
     }
     if (vw_finalize(ctx, Cmap.data(), Wmap.data(), cfg.img_n ? img.data() : nullptr, nullptr) != VW_OK)
       return die(ctx, "vw_finalize");
@@ -291,6 +455,15 @@ int main(int argc, char** argv) {
       std::fclose(f);
     }
     std::printf("Wrote %s in %g s\n", outC.c_str(), tw.toc());
+    if (!save_ckpt(dayid + 1, 0)) {
+      std::fprintf(stderr, "ERROR writing checkpoint %s\n", ckpt_path.c_str());
+      return 1;
+    }
+    if (stop_after > 0 && segs_done >= stop_after && d + 1 < day_count) {
+      std::printf("Stopped after %d segments%s%s\n", segs_done, ckpt_path.empty() ? "" : "; checkpoint ",
+                  ckpt_path.c_str());
+      break;
+    }
   }
   vw_destroy(ctx);
   return 0;

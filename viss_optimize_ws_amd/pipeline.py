@@ -251,6 +251,30 @@ class Pipeline(Stages):
     def set_day(self, day):
         self._check(self._lib.vw_set_day(self._ctx, int(day)))
 
+    def accumulators(self):
+        """Raw per-day accumulators: (C, W, G, GW, wtotal) before normalisation."""
+        C_ = np.empty(self.plan.npix)
+        W = np.empty(self.plan.npix, np.uint32)
+        n = max(self.plan.n_img_pts, 1)
+        G = np.empty(n)
+        GW = np.empty(n, np.uint32)
+        wt = C.c_int64(0)
+        self._check(self._lib.vw_get_accumulators(self._ctx, _ptr(C_, C.c_double), _ptr(W, C.c_uint32),
+                                                  _ptr(G, C.c_double), _ptr(GW, C.c_uint32), C.byref(wt)))
+        k = self.plan.n_img_pts
+        return C_, W, G[:k], GW[:k], wt.value
+
+    def set_accumulators(self, C_, W, G, GW, wtotal):
+        keep, ptrs = [], []
+        for a, dt, ct in ((C_, np.float64, C.c_double), (W, np.uint32, C.c_uint32), (G, np.float64, C.c_double),
+                          (GW, np.uint32, C.c_uint32)):
+            if a is None:
+                ptrs.append(None)
+            else:
+                keep.append(np.ascontiguousarray(a, dt))
+                ptrs.append(_ptr(keep[-1], ct))
+        self._check(self._lib.vw_set_accumulators(self._ctx, *ptrs, int(wtotal)))
+
     def timings(self):
         t = A.vw_timings()
         self._check(self._lib.vw_get_timings(self._ctx, C.byref(t)))
