@@ -160,3 +160,37 @@ def test_cli_argument_errors(tmp_path):
         r = subprocess.run([CLI, "--geometry=circular", "--sky=lognormal", "--nside=4", "--n_epochs=4",
                             f"--out_dir={tmp_path}/o"], capture_output=True, text=True)
         assert r.returncode == 1 and "vw_create" in r.stderr
+
+
+def test_combine_tool_sums_days_in_order(tmp_path):
+    # offline accumulation of per-day float32 maps (combine.py:42-126): day
+    # order by the number in C<n>day..., float32 adds, optional previous sum
+    import subprocess
+
+    tool = os.path.join(ROOT, "viss_optimize_ws_amd", "bin", "vissws_combine")
+    rng = np.random.default_rng(4)
+    maps = {n: rng.normal(0, 1e3, 500).astype(np.float32) for n in (1, 2, 10)}
+    for n, m in maps.items():
+        m.tofile(tmp_path / f"C{n}day30M.bin")
+    r = subprocess.run([tool, f"--dir={tmp_path}", "--pattern=C*day30M.bin"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    exp = np.zeros(500, np.float32)
+    for n in (1, 2, 10):
+        exp += maps[n]
+    out = np.fromfile(tmp_path / "accumulated_sum.bin", np.float32)
+    assert np.array_equal(out, exp)
+    meta = (tmp_path / "accumulated_sum_metadata.txt").read_text()
+    assert "Number of files processed: 3" in meta
+    assert meta.index("C1day") < meta.index("C2day") < meta.index("C10day")
+    prev = (np.arange(500) * 0.5).astype(np.float32)
+    prev.tofile(tmp_path / "prev.bin")
+    r = subprocess.run([tool, f"--dir={tmp_path}", "--pattern=C*day30M.bin", f"--previous_sum={tmp_path}/prev.bin",
+                        f"--output={tmp_path}/acc2.bin"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    exp2 = prev.copy()
+    for n in (1, 2, 10):
+        exp2 += maps[n]
+    assert np.array_equal(np.fromfile(tmp_path / "acc2.bin", np.float32), exp2)
+    np.zeros(3, np.float32).tofile(tmp_path / "C3day30M.bin")
+    r = subprocess.run([tool, f"--dir={tmp_path}", "--pattern=C*day30M.bin"], capture_output=True, text=True)
+    assert r.returncode == 1 and "size mismatch" in r.stderr
