@@ -33,12 +33,17 @@ WORKLOAD_NAME = ("HEALPix NESTED Nside=128 (196,608 px, 768 tiles), 16 antennas 
                  "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 512x512 lm image")
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flops of one evaluated Viss term in the plain form as executed (fma =
-# 2; rint and the table index are not counted): phase u*l + v*m + w*(n-1)
-# 1 mul + 2 fma = 5; reduction x - rint(x), *256, - k = 3; y = f*2pi/256 in
-# two parts 1 mul + 1 fma = 3; z = y^2 1; sin poly 2 fma + mul + fma = 7;
-# cos poly 3 fma = 6; angle addition 2 mul + 2 fma = 6; accumulation
-# 2 fma = 4.
-VISS_TERM_FLOPS_PLAIN = 35
+# 2; the table index is not counted): phase u*l + v*m + w*(n-1) on operands
+# pre-scaled by 1024, 1 mul + 2 fma = 5; split t + M - M, t - k = 3;
+# y = f*2pi/1024 1; z = y^2 1; sin fma + mul + fma = 5; cos 2 fma = 4;
+# rotation by the table entry 2 mul + 2 fma = 6; accumulation 2 fma = 4.
+VISS_TERM_FLOPS_PLAIN = 29
+VISS_TERM_FP64_INSTR = 19
+# recon term, plain form: phase 1 mul + 1 fma = 3, then as Viss from the split
+# on (3 + 1 + 1 + 5 + 4 + 6 + 4) = 27 flops, 18 FP64 instructions
+RECON_TERM_FLOPS_PLAIN = 27
+RECON_TERM_FP64_INSTR = 18
+FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # one FP64 lane instruction per FMA slot
 
 
 def _cfg(precision):
@@ -52,9 +57,14 @@ def _plan_counts(pipe):
     (non-culled) Viss terms, and the final maps for error statements."""
     per_seg = []
     TP = pipe.cfg.tile_pix
-    npix = pipe.plan.npix
-    tile_sizes = np.full(pipe.plan.ntile, TP, np.int64)
-    tile_sizes[-1] = npix - TP * (pipe.plan.ntile - 1)
+
+    def sizes(n, nt):
+        t = np.full(nt, TP, np.int64)
+        t[-1] = n - TP * (nt - 1)
+        return t
+
+    tile_sizes = sizes(pipe.plan.npix, pipe.plan.ntile)
+    img_sizes = sizes(pipe.plan.n_img_pts, pipe.plan.n_img_tiles) if pipe.plan.n_img_pts else None
     pipe.reset_accumulators()
     for t0, tl in pipe.schedule():
         pipe.gen_segment(t0, tl)
@@ -63,7 +73,16 @@ def _plan_counts(pipe):
         vh = pipe.viss_half()
         pipe.conj_fill()
         info = pipe.recon_segment()
-        per_seg.append(dict(n_half=info["n_half"], nkeys=info["nkeys"], evaluated=evaluated, vraw=vh))
+        # recon terms evaluated: (output tile, group) pairs whose representative
+        # half row is not hidden from the tile (uv-mirror recon, occ_mode 2)
+        _, keys, _, _, rep = pipe.groups()
+        reps = rep[keys != 0]
+        rec_eval = int(((cls[reps] != 2).astype(np.int64) * tile_sizes[None, :]).sum())
+        if img_sizes is not None:
+            ci = pipe.tile_classes(img=True)
+            rec_eval += int(((ci[reps] != 2).astype(np.int64) * img_sizes[None, :]).sum())
+        per_seg.append(dict(n_half=info["n_half"], nkeys=info["nkeys"], evaluated=evaluated, rec_evaluated=rec_eval,
+                            vraw=vh))
     maps = pipe.finalize()
     return per_seg, maps
 
@@ -159,6 +178,9 @@ def main():
     viss_avg_ms = tim["viss_kernel_ms"] / nl
     eval_per_launch = evaluated / len(per_seg)
     achieved = VISS_TERM_FLOPS_PLAIN * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
+    rec_eval_step = sum(s_["rec_evaluated"] for s_ in per_seg)
+    rec_ms = tim["recon_kernel_ms"] / args.steps
+    rec_achieved = RECON_TERM_FLOPS_PLAIN * rec_eval_step / (rec_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -187,6 +209,22 @@ def main():
             "flops_per_term": VISS_TERM_FLOPS_PLAIN,
             "evaluated_terms_per_launch": int(eval_per_launch),
             "avg_launch_ms": round(viss_avg_ms, 4),
+            "fp64_instr_issue_frac": round(VISS_TERM_FP64_INSTR * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
+                                           / FP64_INSTR_PEAK_T, 4),
+        },
+        "roofline_recon": {
+            "bound": "mfma",
+            "pipe": "FP64 vector ALU",
+            "kernel": "k_recon (plain, uv mirror, representative classes)",
+            "achieved": round(rec_achieved, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(rec_achieved / FP64_PEAK_TFLOPS, 4),
+            "flops_per_term": RECON_TERM_FLOPS_PLAIN,
+            "evaluated_terms_per_step": rec_eval_step,
+            "kernel_ms_per_step": round(rec_ms, 4),
+            "fp64_instr_issue_frac": round(RECON_TERM_FP64_INSTR * rec_eval_step / (rec_ms * 1e-3) / 1e12
+                                           / FP64_INSTR_PEAK_T, 4),
         },
     }
     if not args.no_mixed:

@@ -145,7 +145,7 @@ def test_pipeline_mixed_c1():
 @pytest.mark.parametrize("recon_mode", [0, 1])
 def test_large_segment_groups(recon_mode):
     # 16 antennas x 40 epochs = 4,800 half rows (9,600 full rows) in one segment:
-    # exercises the multi-workgroup stable ranking instead of the LDS sort
+    # a multi-block stable ranking with many equal keys
     cfg = config(nside=8, n_ant=16, n_epochs=40, segs=1, img_n=0, recon_mode=recon_mode, freq_hz=3e7)
     (Cg, Wg, _, _), (Co, Wo, _, _) = run_both(cfg, _sky(12 * 8 * 8, seed=9), _seg_checks(PLAIN_TOL))
     assert np.array_equal(Wg, Wo)

@@ -555,13 +555,15 @@ int enq_group(Ctx* c) {
   KC();
   k_keys<<<blocks_for(nr), 256, 0, c->st>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
-  if (nr <= VW_SORT_SMALL) {
-    k_sort_small<<<1, 1024, 0, c->st>>>(c->d_keys, nr, c->d_idx_sorted, c->d_keys_sorted);
-    KC();
-  } else {
+  {
+    /* stable rank by (key, row): a grid of (256 rows) x (jblk rows) compare
+     * tiles sized for about 2,048 workgroups, then a scatter */
     HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->st));
-    dim3 grid(blocks_for(nr), (unsigned)((nr + VW_RANK_JBLK - 1) / VW_RANK_JBLK));
-    k_rank_count<<<grid, 256, 0, c->st>>>(c->d_keys, nr, c->d_rank);
+    const int64_t gx = blocks_for(nr);
+    const int64_t gy = std::max<int64_t>(1, std::min<int64_t>(gx, (2048 + gx - 1) / gx));
+    const int64_t jblk = ((nr + gy - 1) / gy + 255) / 256 * 256;
+    dim3 grid((unsigned)gx, (unsigned)((nr + jblk - 1) / jblk));
+    k_rank_count<<<grid, 256, 0, c->st>>>(c->d_keys, nr, jblk, c->d_rank);
     KC();
     k_rank_scatter<<<blocks_for(nr), 256, 0, c->st>>>(c->d_keys, c->d_rank, nr, c->d_idx_sorted, c->d_keys_sorted);
     KC();

@@ -549,55 +549,17 @@ __global__ void k_keys(const RowRec* __restrict__ rows, int64_t nr, int P, int f
   keys[r] = vw_grid_key(sg * rec.u, sg * rec.v, &sp, full ? 0 : 1);
 }
 
-/* Stable order by (key, row index).  Up to SORT_SMALL rows: one workgroup,
- * bitonic network over (key, index) pairs held in LDS (padding sorts last).
- * Larger: all-pairs rank counted over a 2-D grid of (row block, key block)
- * with integer atomics (exact, order-independent), then a scatter. */
-#define VW_SORT_SMALL 4096
-__global__ __launch_bounds__(1024) void k_sort_small(const int64_t* __restrict__ keys, int64_t nr,
-                                                     int32_t* __restrict__ idx_sorted,
-                                                     int64_t* __restrict__ keys_sorted) {
-  __shared__ int64_t sk[VW_SORT_SMALL];
-  __shared__ int32_t si[VW_SORT_SMALL];
-  const int t = threadIdx.x;
-  int npad = 2;
-  while (npad < nr) npad <<= 1;
-  for (int i = t; i < npad; i += 1024) {
-    sk[i] = i < nr ? keys[i] : INT64_MAX;
-    si[i] = i;
-  }
-  __syncthreads();
-  for (int k = 2; k <= npad; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < npad; i += 1024) {
-        int x = i ^ j;
-        if (x > i) {
-          int64_t ka = sk[i], kb = sk[x];
-          int32_t ia = si[i], ib = si[x];
-          bool gt = ka > kb || (ka == kb && ia > ib);
-          bool up = (i & k) == 0;
-          if (gt == up) {
-            sk[i] = kb; sk[x] = ka;
-            si[i] = ib; si[x] = ia;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = t; i < nr; i += 1024) {
-    idx_sorted[i] = si[i];
-    keys_sorted[i] = sk[i];
-  }
-}
-
-#define VW_RANK_JBLK 2048
-__global__ __launch_bounds__(256) void k_rank_count(const int64_t* __restrict__ keys, int64_t nr,
+/* Stable order by (key, row index): all-pairs rank counted over a 2-D grid
+ * of (256-row block, jblk-row block) compare tiles with integer atomics
+ * (exact, order-independent), then a scatter.  The grid is sized for about
+ * 2,048 workgroups, so a segment of a few thousand rows costs a few
+ * microseconds instead of a one-workgroup sort. */
+__global__ __launch_bounds__(256) void k_rank_count(const int64_t* __restrict__ keys, int64_t nr, int64_t jblk,
                                                     int32_t* __restrict__ rank) {
   __shared__ int64_t sk[256];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t j0 = (int64_t)blockIdx.y * VW_RANK_JBLK;
-  const int64_t j1 = j0 + VW_RANK_JBLK < nr ? j0 + VW_RANK_JBLK : nr;
+  const int64_t j0 = (int64_t)blockIdx.y * jblk;
+  const int64_t j1 = j0 + jblk < nr ? j0 + jblk : nr;
   const int64_t ki = i < nr ? keys[i] : 0;
   int32_t cnt = 0;
   for (int64_t base = j0; base < j1; base += 256) {
