@@ -104,8 +104,11 @@ VW_HD void vw_sincos_turn(double r, double* s_out, double* c_out) {
  * by vw_sincos_tab_build with vw_sincos_turn, identically on host and
  * device.  Error within ~2 ulp of sin/cos; 15 operations plus one table
  * read. */
-#define VW_TAB_N 1024
-#define VW_TAB_SCALE 1024.0
+#ifndef VW_TAB_LOG2
+#define VW_TAB_LOG2 10
+#endif
+#define VW_TAB_N (1 << VW_TAB_LOG2)
+#define VW_TAB_SCALE ((double)VW_TAB_N)
 VW_HD void vw_sincos_tab_build(int k, double* tab) {
   vw_sincos_turn((double)k / (double)VW_TAB_N, &tab[2 * k + 1], &tab[2 * k]);
 }
@@ -128,13 +131,22 @@ VW_HD double vw_tab_split(double t, int* idx) {
   return t - k;
 }
 
-/* sin and cos of y = 2*pi*f/1024, |f| <= 1/2 */
+/* sin and cos of y = 2*pi*f/N, |f| <= 1/2 (smaller tables add the y^7 and
+ * y^6 terms) */
 VW_HD void vw_tab_poly(double f, double* sy, double* cy) {
-  double y = f * 0x1.921fb54442d18p-8; /* 2*pi/1024 */
+  double y = f * (0x1.921fb54442d18p+2 / VW_TAB_SCALE); /* 2*pi/N, exact power-of-two scaling */
   double z = y * y;
+#if VW_TAB_LOG2 < 9
+  double ps = fma(fma(z, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7), z, -0x1.5555555555555p-3);
+#else
   double ps = fma(z, 0x1.1111111111111p-7, -0x1.5555555555555p-3); /* 1/120, -1/6 */
+#endif
   *sy = fma(y * z, ps, y);
+#if VW_TAB_LOG2 < 10
+  *cy = fma(z, fma(z, fma(z, -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5), -0.5), 1.0);
+#else
   *cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0); /* 1/24 */
+#endif
 }
 
 /* rotation by the table entry (ec, es) */
