@@ -46,6 +46,20 @@ RECON_TERM_FP64_INSTR = 18
 FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # one FP64 lane instruction per FMA slot
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_nside128_16ant_512ep.json")
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of a kernel at this workload from the committed
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), or
+    None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return int(json.load(f)["kernels"][kernel]["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _cfg(precision):
     from viss_optimize_ws_amd import default_config
 
@@ -205,7 +219,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": _traffic("k_viss_plain"),
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "flops_per_term": VISS_TERM_FLOPS_PLAIN,
             "evaluated_terms_per_launch": int(eval_per_launch),
             "avg_launch_ms": round(viss_avg_ms, 4),
@@ -221,6 +236,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(rec_achieved / FP64_PEAK_TFLOPS, 4),
             "flops_per_term": RECON_TERM_FLOPS_PLAIN,
+            "traffic": _traffic("k_recon_plain"),
             "evaluated_terms_per_step": rec_eval_step,
             "kernel_ms_per_step": round(rec_ms, 4),
             "fp64_instr_issue_frac": round(RECON_TERM_FP64_INSTR * rec_eval_step / (rec_ms * 1e-3) / 1e12
