@@ -35,14 +35,14 @@ FP64_PEAK_TFLOPS = 78.6
 # FP64 flops of one evaluated Viss term in the plain form as executed (fma =
 # 2; the table index is not counted): phase u*l + v*m + w*(n-1) on operands
 # pre-scaled by 1024, 1 mul + 2 fma = 5; split t + M - M, t - k = 3;
-# y = f*2pi/1024 1; z = y^2 1; sin fma + mul + fma = 5; cos 2 fma = 4;
+# z = f^2 1; sin f*(a + z*(b + z*c)) 2 fma + mul = 5; cos 2 fma = 4;
 # rotation by the table entry 2 mul + 2 fma = 6; accumulation 2 fma = 4.
-VISS_TERM_FLOPS_PLAIN = 29
-VISS_TERM_FP64_INSTR = 19
+VISS_TERM_FLOPS_PLAIN = 28
+VISS_TERM_FP64_INSTR = 18
 # recon term, plain form: phase 1 mul + 1 fma = 3, then as Viss from the split
-# on (3 + 1 + 1 + 5 + 4 + 6 + 4) = 27 flops, 18 FP64 instructions
-RECON_TERM_FLOPS_PLAIN = 27
-RECON_TERM_FP64_INSTR = 18
+# on (3 + 1 + 5 + 4 + 6 + 4) = 26 flops, 17 FP64 instructions
+RECON_TERM_FLOPS_PLAIN = 26
+RECON_TERM_FP64_INSTR = 17
 FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # one FP64 lane instruction per FMA slot
 
 

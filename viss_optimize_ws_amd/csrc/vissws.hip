@@ -269,8 +269,14 @@ struct VissShape {
 };
 /* half rows per wave: as many accumulators as fit without spilling
  * (the counting variants carry one more register per row) */
+#ifndef VW_VISS_RPW_PLAIN
+#define VW_VISS_RPW_PLAIN 16
+#endif
+#ifndef VW_VISS_RPW_MIXED
+#define VW_VISS_RPW_MIXED 16
+#endif
 constexpr int viss_rpw(int /*TP*/, bool mixed, bool count) {
-  return (mixed ? 32 : 16) / (count ? 2 : 1);
+  return (mixed ? VW_VISS_RPW_MIXED : VW_VISS_RPW_PLAIN) / (count ? 2 : 1);
 }
 
 VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {

@@ -362,7 +362,16 @@ __device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const dou
 }
 
 template <int TP, int RPW, bool MIXED, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_viss(VissArgs a) {
+/* occupancy: the plain form keeps 16 FP64 row sums per lane at 3 waves per
+ * SIMD; the mixed form (16 FP32 row sums) runs 5 (measured best) */
+#ifndef VW_VISS_WAVES_PLAIN
+#define VW_VISS_WAVES_PLAIN 3
+#endif
+#ifndef VW_VISS_WAVES_MIXED
+#define VW_VISS_WAVES_MIXED 5
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_VISS_WAVES_MIXED
+                                                                            : VW_VISS_WAVES_PLAIN))) void k_viss(VissArgs a) {
   /* a tile is walked in sub-tiles of 256 pixels (4 per lane), so the
    * register footprint does not grow with the tile size */
   constexpr int PPL = 4;
@@ -825,8 +834,15 @@ struct ReconLane {
   __device__ double value(int k) const { return MIXED ? f[k].value() : d[k]; }
 };
 
+#ifndef VW_RECON_WAVES_PLAIN
+#define VW_RECON_WAVES_PLAIN 4
+#endif
+#ifndef VW_RECON_WAVES_MIXED
+#define VW_RECON_WAVES_MIXED 5
+#endif
 template <int PPL, bool MIXED, int KIND>
-__global__ __launch_bounds__(256) void k_recon(ReconArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_RECON_WAVES_MIXED
+                                                                            : VW_RECON_WAVES_PLAIN))) void k_recon(ReconArgs a) {
   __shared__ double2 tab[MIXED ? 1 : VW_TAB_N * VW_TAB_REPL];
   __shared__ double gring[4][2 * 64]; /* per wave: two blocks of 16 group records */
   if (!MIXED) {

@@ -102,7 +102,7 @@ VW_HD void vw_sincos_turn(double r, double* s_out, double* c_out) {
  * cos(y) = 1 - y^2/2 + y^4/24 (omitted term < 1.2e-18), then one rotation
  * by the table entry (cos, sin)(2*pi*(k mod 1024)/1024).  The table is built
  * by vw_sincos_tab_build with vw_sincos_turn, identically on host and
- * device.  Error within ~2 ulp of sin/cos; 15 operations plus one table
+ * device.  Error within ~2 ulp of sin/cos; 14 operations plus one table
  * read. */
 #ifndef VW_TAB_LOG2
 #define VW_TAB_LOG2 10
@@ -131,22 +131,33 @@ VW_HD double vw_tab_split(double t, int* idx) {
   return t - k;
 }
 
-/* sin and cos of y = 2*pi*f/N, |f| <= 1/2 (smaller tables add the y^7 and
- * y^6 terms) */
+/* sin and cos of 2*pi*f/N, |f| <= 1/2, as polynomials in f (Taylor
+ * coefficients of sin/cos(a f), a = 2*pi/N, rounded to double; the values
+ * for N = 256 are scaled by exact powers of two for larger tables):
+ *   sin = f (a + f^2 (-a^3/6 + f^2 a^5/120 [- f^2 a^7/5040]))
+ *   cos = 1 + f^2 (-a^2/2 + f^2 (a^4/24 [- f^2 a^6/720]))
+ * the bracketed terms only for smaller tables (their size < 1.2e-18 at
+ * N = 1024).  Working in f saves the product y = a f. */
+#if VW_TAB_LOG2 < 8 || VW_TAB_LOG2 > 10
+#error "table size 2^8 .. 2^10"
+#endif
+#define VW_TAB_SH(k) ((double)(1ULL << ((k) * (VW_TAB_LOG2 - 8))))
 VW_HD void vw_tab_poly(double f, double* sy, double* cy) {
-  double y = f * (0x1.921fb54442d18p+2 / VW_TAB_SCALE); /* 2*pi/N, exact power-of-two scaling */
-  double z = y * y;
+  double z = f * f;
 #if VW_TAB_LOG2 < 9
-  double ps = fma(fma(z, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7), z, -0x1.5555555555555p-3);
+  double ps = fma(fma(z, -0x1.32d2cce62bd86p-50 / VW_TAB_SH(7), 0x1.466bc6775aae2p-34 / VW_TAB_SH(5)), z,
+                  -0x1.4abbce625be53p-19 / VW_TAB_SH(3));
 #else
-  double ps = fma(z, 0x1.1111111111111p-7, -0x1.5555555555555p-3); /* 1/120, -1/6 */
+  double ps = fma(z, 0x1.466bc6775aae2p-34 / VW_TAB_SH(5), -0x1.4abbce625be53p-19 / VW_TAB_SH(3));
 #endif
-  *sy = fma(y * z, ps, y);
+  *sy = f * fma(z, ps, 0x1.921fb54442d18p-6 / VW_TAB_SH(1));
 #if VW_TAB_LOG2 < 10
-  *cy = fma(z, fma(z, fma(z, -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5), -0.5), 1.0);
+  double pc = fma(fma(z, -0x1.55d3c7e3cbffap-42 / VW_TAB_SH(6), 0x1.03c1f081b5ac4p-26 / VW_TAB_SH(4)), z,
+                  -0x1.3bd3cc9be45dep-12 / VW_TAB_SH(2));
 #else
-  *cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0); /* 1/24 */
+  double pc = fma(z, 0x1.03c1f081b5ac4p-26 / VW_TAB_SH(4), -0x1.3bd3cc9be45dep-12 / VW_TAB_SH(2));
 #endif
+  *cy = fma(z, pc, 1.0);
 }
 
 /* rotation by the table entry (ec, es) */
