@@ -27,10 +27,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "pixel-baseline Gterms/s Viss+recon, plain FP64 vs mixed at stated FP64 err"
-WORKLOAD = dict(nside=128, n_ant=16, n_epochs=512, freq_hz=3e7, segs=16, img_n=512, tile_pix=256)
-WORKLOAD_NAME = ("HEALPix NESTED Nside=128 (196,608 px, 768 tiles), 16 antennas / 120 baselines on seeded "
+WORKLOADS = {
+    # the headline configuration (default)
+    "nside128": (dict(nside=128, n_ant=16, n_epochs=512, freq_hz=3e7, segs=16, img_n=512, tile_pix=256),
+                 "HEALPix NESTED Nside=128 (196,608 px, 768 tiles), 16 antennas / 120 baselines on seeded "
                  "circular orbits r in [6900,7100] km, 512 epochs x 10 s, 30 MHz, 16 segments, "
-                 "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 512x512 lm image")
+                 "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 512x512 lm image"),
+    # the mixed-precision configuration: 507,904 half visibilities, 1024^2 image
+    "nside256": (dict(nside=256, n_ant=32, n_epochs=1024, freq_hz=1e7, segs=16, img_n=1024, tile_pix=256),
+                 "HEALPix NESTED Nside=256 (786,432 px, 3,072 tiles), 32 antennas / 496 baselines on seeded "
+                 "circular orbits r in [6900,7100] km, 1,024 epochs x 10 s, 10 MHz, 16 segments, "
+                 "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 1024x1024 lm image"),
+}
+WORKLOAD, WORKLOAD_NAME = WORKLOADS["nside128"]
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flops of one evaluated Viss term in the plain form as executed (fma =
 # 2; the table index is not counted): phase u*l + v*m + w*(n-1) on operands
@@ -46,15 +55,16 @@ RECON_TERM_FP64_INSTR = 17
 FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # one FP64 lane instruction per FMA slot
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_nside128_16ant_512ep.json")
+TRAFFIC_FILES = {"nside128": os.path.join(ROOT, "profiles", "pmc_traffic_nside128_16ant_512ep.json"),
+                 "nside256": os.path.join(ROOT, "profiles", "pmc_traffic_nside256_32ant_1024ep.json")}
 
 
-def _traffic(kernel):
+def _traffic(workload, kernel):
     """HBM bytes per launch of a kernel at this workload from the committed
     FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), or
     None when that summary is absent."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[workload]) as f:
             return int(json.load(f)["kernels"][kernel]["bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
@@ -118,10 +128,12 @@ def _time_steps(pipe, steps, warmup):
     return (t1 - t0) / steps, pipe.timings()
 
 
-def _cpu_baseline(sky, threads):
-    """CPU FP64 oracle on a bounded sample: the first segment in full (Viss of
-    all its half rows, reconstruction of every HEALPix pixel and image
-    point)."""
+def _cpu_baseline(sky, threads, vraw0, full_segment):
+    """CPU FP64 oracle on a bounded sample of the first segment.  full_segment:
+    Viss of all its half rows and reconstruction of every HEALPix pixel and
+    image point.  Otherwise (large workloads): Viss of the first 2,048 half
+    rows, then, from the segment's GPU half visibilities, grouping and the
+    reconstruction of the first 16,384 HEALPix pixels and image points."""
     from oracle.oracle import Oracle
 
     cfg = _cfg(0)
@@ -129,14 +141,32 @@ def _cpu_baseline(sky, threads):
         o.set_sky(sky)
         t0, tl = o.schedule()[0]
         o.gen_segment(t0, tl)
-        s = time.perf_counter()
-        o.viss_half()
-        o.conj_fill()
-        info = o.recon_segment()
-        dt = time.perf_counter() - s
-        terms = o.plan.npix * info["n_half"] + (o.plan.npix + o.plan.n_img_pts) * info["nkeys"]
-    return dict(value=terms / dt / 1e9, unit="Gterms/s", cores=threads, kind="port",
-                sample=f"segment 1 of 16 in full: {terms:.3e} terms in {dt:.2f} s (oracle, OpenMP FP64)")
+        if full_segment:
+            s = time.perf_counter()
+            o.viss_half()
+            o.conj_fill()
+            info = o.recon_segment()
+            dt = time.perf_counter() - s
+            terms = o.plan.npix * info["n_half"] + (o.plan.npix + o.plan.n_img_pts) * info["nkeys"]
+            sample = f"segment 1 of 16 in full: {terms:.3e} terms in {dt:.2f} s (oracle, OpenMP FP64)"
+        else:
+            nr, npt = min(2048, o.nh), 16384
+            s = time.perf_counter()
+            o.viss_half(0, nr)
+            dv = time.perf_counter() - s
+            o.set_viss_raw(vraw0)
+            o.conj_fill()
+            info = o.group_segment()
+            s = time.perf_counter()
+            o.recon_points(0, np.arange(min(npt, o.plan.npix)))
+            o.recon_points(1, np.arange(min(npt, o.plan.n_img_pts)))
+            dr = time.perf_counter() - s
+            dt = dv + dr
+            terms = o.plan.npix * nr + (min(npt, o.plan.npix) + min(npt, o.plan.n_img_pts)) * info["nkeys"]
+            sample = (f"segment 1 of 16: Viss of {nr} half rows ({dv:.2f} s) + recon of {npt} HEALPix pixels and "
+                      f"{npt} image points over its {info['nkeys']} keys ({dr:.2f} s): {terms:.3e} terms "
+                      "(oracle, OpenMP FP64)")
+    return dict(value=terms / dt / 1e9, unit="Gterms/s", cores=threads, kind="port", sample=sample)
 
 
 def _copy_calibration(mib):
@@ -162,12 +192,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nside128",
+                    help="nside128 (default, the headline configuration) or nside256")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.copy_calib:
         _copy_calibration(args.copy_calib)
         return
+    global WORKLOAD, WORKLOAD_NAME
+    WORKLOAD, WORKLOAD_NAME = WORKLOADS[args.workload]
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one MI355X)",
               file=sys.stderr)
@@ -219,8 +253,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": _traffic("k_viss_plain"),
-            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+            "traffic": _traffic(args.workload, "k_viss_plain"),
+            "traffic_source": os.path.relpath(TRAFFIC_FILES[args.workload], ROOT),
             "flops_per_term": VISS_TERM_FLOPS_PLAIN,
             "evaluated_terms_per_launch": int(eval_per_launch),
             "avg_launch_ms": round(viss_avg_ms, 4),
@@ -236,7 +270,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(rec_achieved / FP64_PEAK_TFLOPS, 4),
             "flops_per_term": RECON_TERM_FLOPS_PLAIN,
-            "traffic": _traffic("k_recon_plain"),
+            "traffic": _traffic(args.workload, "k_recon_plain"),
             "evaluated_terms_per_step": rec_eval_step,
             "kernel_ms_per_step": round(rec_ms, 4),
             "fp64_instr_issue_frac": round(RECON_TERM_FP64_INSTR * rec_eval_step / (rec_ms * 1e-3) / 1e12
@@ -262,7 +296,7 @@ def main():
                         "viss_kernel_avg_ms": round(tim_m["viss_kernel_ms"] / max(tim_m["n_viss_launch"], 1), 4)}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = _cpu_baseline(sky, threads)
+        out["cpu_baseline"] = _cpu_baseline(sky, threads, per_seg[0]["vraw"], args.workload == "nside128")
     print(json.dumps(out), flush=True)
 
 

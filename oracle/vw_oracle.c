@@ -617,8 +617,28 @@ int vo_group_segment(void* ctx, vw_seg_info* info) {
   return VW_OK;
 }
 
-/* recon of one output point for the current segment */
-static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint32_t* wseg) {
+/* effective mask from the row's class for the point's tile (rowcls[h],
+ * computed once per output tile), falling back to the per-point predicate
+ * on mixed tiles -- the same value as eff_mask */
+static int eff_mask_cls(const vo_ctx* c, const uint8_t* rowcls, int64_t h, int img, int64_t pt) {
+  int cls = rowcls[h];
+  if (cls == VW_CLS_VISIBLE) return 1;
+  if (cls == VW_CLS_HIDDEN) return 0;
+  if (img) return predicate(c, h, c->gl[pt], c->gm[pt], c->gn[pt]);
+  return predicate(c, h, c->l[pt], c->m[pt], c->n[pt]);
+}
+
+/* classes of every half row of the segment for one output tile */
+static void tile_row_classes(const vo_ctx* c, int img, int64_t tile, uint8_t* rowcls) {
+  const double* cone = img ? c->gcone : c->cone;
+  int64_t ntile = img ? c->ngt : c->ntile;
+  for (int64_t h = 0; h < c->nh; ++h) rowcls[h] = (uint8_t)pair_class(c, h, cone, ntile, tile);
+}
+
+/* recon of one output point for the current segment; rowcls holds the row
+ * classes of the point's tile; gu/gv the cell centres of the groups */
+static void recon_point(const vo_ctx* c, int img, int64_t pt, const uint8_t* rowcls, const double* gu,
+                        const double* gv, double* cseg, uint32_t* wseg) {
   double l = img ? c->gl[pt] : c->l[pt];
   double m = img ? c->gm[pt] : c->m[pt];
   double n = img ? c->gn[pt] : c->n[pt];
@@ -627,11 +647,8 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
   uint32_t w = 0;
   int mirror = c->cfg.recon_mode == VW_RECON_MIRROR;
   for (int64_t q = 0; q < c->nuniq; ++q) {
-    int64_t key = c->gkeys[q];
-    if (key == 0) continue;
-    double ugu, vgu;
-    vw_key_uv(key, &c->sp, &ugu, &vgu);
-    const double t = vw_recon_phase(ugu, vgu, lp, mp) * VW_TAB_SCALE;
+    if (c->gkeys[q] == 0) continue;
+    const double t = vw_recon_phase(gu[q], gv[q], lp, mp) * VW_TAB_SCALE;
     double sx, cx; /* sin, cos of the phase, evaluated where the term is used */
     if (!c->cfg.blockage) {
       vw_sincos_tab(t, g_tab, &sx, &cx);
@@ -641,7 +658,7 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
     int32_t s0 = c->goff[q], s1 = c->goff[q + 1], L = s1 - s0;
     if (mirror && c->cfg.occ_mode == 2) {
       int64_t h = c->grep[q];
-      if (eff_mask(c, h, img, pt)) {
+      if (eff_mask_cls(c, rowcls, h, img, pt)) {
         vw_sincos_tab(t, g_tab, &sx, &cx);
         acc = vw_recon_add(acc, c->vh[h].re, c->vh[h].im, sx, cx);
         w += 1;
@@ -661,7 +678,7 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
       int32_t pos = (np < 0) ? kk : posv[kk];
       int64_t h; int neg;
       row_half(c, c->idx_sorted[pos], &h, &neg);
-      if (!eff_mask(c, h, img, pt)) continue;
+      if (!eff_mask_cls(c, rowcls, h, img, pt)) continue;
       double vr = c->vh[h].re, vi = neg ? -c->vh[h].im : c->vh[h].im;
       if (isfinite(vr) && isfinite(vi)) { sr = sr + vr; si = si + vi; ++cnt; }
     }
@@ -676,13 +693,58 @@ static void recon_point(const vo_ctx* c, int img, int64_t pt, double* cseg, uint
   *wseg = w;
 }
 
+/* group cell centres (wavelengths) of the current segment */
+static void group_uv(const vo_ctx* c, double* gu, double* gv) {
+  for (int64_t q = 0; q < c->nuniq; ++q) {
+    gu[q] = gv[q] = 0.0;
+    if (c->gkeys[q] != 0) vw_key_uv(c->gkeys[q], &c->sp, &gu[q], &gv[q]);
+  }
+}
+
+/* recon of the points [first, first + n) of one output set (img), tile by
+ * tile: the row classes of a tile are computed once for all its points */
+static void recon_range(const vo_ctx* c, int img, int64_t n, const double* gu, const double* gv, double* acc,
+                        uint32_t* wacc, int accumulate) {
+  int64_t ntile = (n + c->TP - 1) / c->TP;
+#pragma omp parallel
+  {
+    uint8_t* rowcls = (uint8_t*)xcalloc((size_t)c->nh, 1);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t t = 0; t < ntile; ++t) {
+      tile_row_classes(c, img, t, rowcls);
+      int64_t p1 = (t + 1) * c->TP < n ? (t + 1) * c->TP : n;
+      for (int64_t p = t * c->TP; p < p1; ++p) {
+        double cs; uint32_t ws;
+        recon_point(c, img, p, rowcls, gu, gv, &cs, &ws);
+        if (accumulate) { acc[p] = acc[p] + cs; wacc[p] += ws; }
+        else { acc[p] = cs; wacc[p] = ws; }
+      }
+    }
+    free(rowcls);
+  }
+}
+
 int vo_recon_points(void* ctx, int32_t img, const int64_t* pts, int64_t npts, double* cseg, uint32_t* wseg) {
   vo_ctx* c = (vo_ctx*)ctx;
   if (!c->have_groups) return fail(c, VW_ERR_STATE, "need groups");
   int64_t lim = img ? c->ngp : c->npix;
   for (int64_t k = 0; k < npts; ++k) if (pts[k] < 0 || pts[k] >= lim) return fail(c, VW_ERR_ARG, "point index");
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int64_t k = 0; k < npts; ++k) recon_point(c, img, pts[k], &cseg[k], &wseg[k]);
+  double* gu = (double*)xcalloc((size_t)c->nuniq, sizeof(double));
+  double* gv = (double*)xcalloc((size_t)c->nuniq, sizeof(double));
+  group_uv(c, gu, gv);
+#pragma omp parallel
+  {
+    uint8_t* rowcls = (uint8_t*)xcalloc((size_t)c->nh, 1);
+    int64_t cur = -1;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t k = 0; k < npts; ++k) {
+      int64_t t = pts[k] / c->TP;
+      if (t != cur) { tile_row_classes(c, img, t, rowcls); cur = t; }
+      recon_point(c, img, pts[k], rowcls, gu, gv, &cseg[k], &wseg[k]);
+    }
+    free(rowcls);
+  }
+  free(gu); free(gv);
   return VW_OK;
 }
 
@@ -690,22 +752,12 @@ int vo_recon_segment(void* ctx, vw_seg_info* info) {
   vo_ctx* c = (vo_ctx*)ctx;
   int rc = vo_group_segment(ctx, info);
   if (rc) return rc;
-  if (c->cfg.healpix_out) {
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t p = 0; p < c->npix; ++p) {
-      double cs; uint32_t ws;
-      recon_point(c, 0, p, &cs, &ws);
-      c->Cacc[p] = c->Cacc[p] + cs;
-      c->Wacc[p] += ws;
-    }
-  }
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int64_t k = 0; k < c->ngp; ++k) {
-    double cs; uint32_t ws;
-    recon_point(c, 1, k, &cs, &ws);
-    c->Gacc[k] = c->Gacc[k] + cs;
-    c->GWacc[k] += ws;
-  }
+  double* gu = (double*)xcalloc((size_t)c->nuniq, sizeof(double));
+  double* gv = (double*)xcalloc((size_t)c->nuniq, sizeof(double));
+  group_uv(c, gu, gv);
+  if (c->cfg.healpix_out) recon_range(c, 0, c->npix, gu, gv, c->Cacc, c->Wacc, 1);
+  if (c->ngp) recon_range(c, 1, c->ngp, gu, gv, c->Gacc, c->GWacc, 1);
+  free(gu); free(gv);
   if (!c->cfg.blockage) c->Wtotal += c->nuniq;
   return VW_OK;
 }
