@@ -67,6 +67,9 @@ struct Ctx {
   int64_t cls_cap = 0;       /* bytes                                                    */
   int64_t vtpp = 0;          /* sky tiles per Viss pass                                  */
   int64_t rpart_cap = 0;     /* doubles in each recon partial buffer                     */
+  uint8_t* d_bits = nullptr; /* cone bits per (tile of a pass, antenna frame)            */
+  int64_t nfr_pad = 0;       /* antenna frames of the longest segment, padded            */
+  int64_t bits_cap = 0;      /* bytes of d_bits                                          */
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
   int64_t* d_counts = nullptr;
@@ -455,6 +458,16 @@ int create_device(Ctx* c) {
     }
     c->rpart_cap = rp;
     if ((rc = dalloc(c, &c->d_rpart, (size_t)rp)) || (rc = dalloc(c, &c->d_rwpart, (size_t)rp))) return rc;
+    /* cone bits for the widest tile pass of either stage */
+    int64_t maxnt = c->vtpp;
+    for (int img = 0; img < 2; ++img) {
+      const int64_t nta = img ? c->ngt : c->ntile;
+      if (nta > 0)
+        for (int64_t w : {c->nh_pad, c->nq_pad}) maxnt = std::max(maxnt, recon_pass_tiles(c, nta, w));
+    }
+    c->nfr_pad = ((int64_t)c->max_tlen * c->A + 63) / 64 * 64;
+    c->bits_cap = maxnt * c->nfr_pad;
+    if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap))) return rc;
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
@@ -489,6 +502,24 @@ int enq_gen(Ctx* c, int t0, int tlen) {
   return VW_OK;
 }
 
+/* class table of a tile pass into the class scratch: cone bits per
+ * (tile, antenna frame), then per (tile, half row) or (tile, group) */
+int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups) {
+  const int64_t nfr = (int64_t)c->tlen * c->A;
+  if (nt * c->nfr_pad > c->bits_cap) return c->fail(VW_ERR_RANGE, "internal: cone bit buffer too small");
+  k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad,
+                                                             c->d_bits);
+  KC();
+  if (groups)
+    k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, c->st>>>(c->d_bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib,
+                                                               c->d_info, c->nq_pad, c->d_cls);
+  else
+    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad,
+                                                              c->d_cls);
+  KC();
+  return VW_OK;
+}
+
 template <int TP, int RPW, bool MIXED, bool COUNT>
 void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
   k_viss<TP, RPW, MIXED, COUNT><<<nb, 256, 0, st>>>(a);
@@ -497,19 +528,18 @@ void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
 int enq_viss(Ctx* c, bool count) {
   if (!c->have_seg || !c->have_sky) return c->fail(VW_ERR_STATE, "vw_viss_half needs vw_set_sky and vw_gen_segment");
   if (count && !c->d_cpart) {
-    int rc;
-    if ((rc = dalloc(c, &c->d_cpart, (size_t)c->nchunk_max * c->nh_pad)) || (rc = dalloc(c, &c->d_counts, (size_t)c->max_nh)))
-      return rc;
+    int rc2;
+    if ((rc2 = dalloc(c, &c->d_cpart, (size_t)c->nchunk_max * c->nh_pad)) || (rc2 = dalloc(c, &c->d_counts, (size_t)c->max_nh)))
+      return rc2;
   }
   const bool mixed = c->cfg.precision == VW_PREC_MIXED;
+  int rc;
   ev_begin(c, ST_VISS);
   for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
     VissShape s = viss_shape(c, c->nh, nt, count);
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
-    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_cone, c->ntile, tile0, nt, c->d_rows, c->d_ant8,
-                                                              c->nh, c->nh_pad, c->d_cls);
-    KC();
+    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false))) return rc;
     VissArgs a{};
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
@@ -607,14 +637,9 @@ int enq_recon_set(Ctx* c, bool img) {
     const int64_t nt = std::min(otpp, ntile_all - tile0);
     const int64_t pt0 = tile0 * c->TP;
     const int64_t np = std::min(nt * c->TP, npts_all - pt0);
-    if (kind == VW_KIND_REP) {
-      k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->gs.gia, c->gs.gib,
-                                                                 c->d_ant8, c->d_info, c->nq_pad, c->d_cls);
-      KC();
-    } else if (kind == VW_KIND_GENERIC) {
-      k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->d_rows, c->d_ant8, c->nh,
-                                                                c->nh_pad, c->d_cls);
-      KC();
+    if (kind != VW_KIND_AVG) {
+      int rc = enq_classes(c, cone, ntile_all, tile0, nt, kind == VW_KIND_REP);
+      if (rc) return rc;
     }
     ReconArgs a{};
     a.l = img ? c->d_gl : c->d_l;
@@ -1051,13 +1076,18 @@ namespace {
 int full_classes(Ctx* c, bool img, std::vector<uint8_t>& t, uint8_t** dev_out) {
   int64_t nt = img ? c->ngt : c->ntile;
   t.assign((size_t)(nt * c->nh_pad), 0);
-  uint8_t* d = nullptr;
+  uint8_t *d = nullptr, *bits = nullptr;
   HC(hipMalloc(&d, t.size()));
-  k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(img ? c->d_gcone : c->d_cone, nt, 0, nt, c->d_rows,
-                                                            c->d_ant8, c->nh, c->nh_pad, d);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipMalloc(&bits, (size_t)(nt * c->nfr_pad));
+  if (e == hipSuccess) {
+    k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, c->st>>>(img ? c->d_gcone : c->d_cone, nt, 0, nt, c->d_ant8,
+                                                               (int64_t)c->tlen * c->A, c->nfr_pad, bits);
+    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, d);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(t.data(), d, t.size(), hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (bits) (void)hipFree(bits);
   if (e != hipSuccess || !dev_out) (void)hipFree(d);
   if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("class table: ") + hipGetErrorString(e));
   if (dev_out) *dev_out = d;

@@ -242,24 +242,34 @@ __global__ void k_seg_full(const double* __restrict__ pos, int T, int A, int P, 
   out[8 * nf + i] = pos[2 * TA + k2];
 }
 
-/* pair class per (tile, half row) for tiles [tile0, tile0+nt) -> bytes
- * [nt][nh_pad]; cone arrays hold all ntile tiles */
-__global__ void k_classify(const double* __restrict__ cone, int64_t ntile, int64_t tile0, int64_t nt,
-                           const RowRec* __restrict__ rows, const double* __restrict__ ant8, int64_t nh,
-                           int64_t nh_pad, uint8_t* __restrict__ cls) {
+/* cone bits of every antenna frame against tiles [tile0, tile0+nt) ->
+ * bytes [nt][nfr_pad] (bit0 all visible, bit1 all hidden); cone arrays hold
+ * all ntile tiles.  A row's class only combines the bits of its two frames,
+ * so the cone tests run once per (tile, frame) instead of twice per
+ * (tile, row): tlen*A frames against tlen*A(A-1)/2 rows. */
+__global__ void k_ant_bits(const double* __restrict__ cone, int64_t ntile, int64_t tile0, int64_t nt,
+                           const double* __restrict__ ant8, int64_t nfr, int64_t nfr_pad, uint8_t* __restrict__ bits) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nt * nfr_pad) return;
+  int64_t tr = k / nfr_pad, f = k - tr * nfr_pad;
+  if (f >= nfr) { bits[k] = 0; return; }
+  int64_t t = tile0 + tr;
+  const double* q = ant8 + f * 8;
+  bits[k] = (uint8_t)vw_cone_bits(q[4], q[5], q[6], cone[t], cone[ntile + t], cone[2 * ntile + t],
+                                  cone[3 * ntile + t], cone[4 * ntile + t], q[7]);
+}
+
+/* pair class per (tile, half row) for the nt tiles of a pass -> bytes
+ * [nt][nh_pad] (rows past nh: hidden), from the frame bits */
+__global__ void k_classify(const uint8_t* __restrict__ bits, int64_t nfr_pad, int64_t nt,
+                           const RowRec* __restrict__ rows, int64_t nh, int64_t nh_pad, uint8_t* __restrict__ cls) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nt * nh_pad) return;
   int64_t tr = k / nh_pad, h = k - tr * nh_pad;
-  int64_t t = tile0 + tr;
   if (h >= nh) { cls[k] = VW_CLS_HIDDEN; return; }
-  RowRec r = rows[h];
-  const double* qa = ant8 + (int64_t)r.ia * 8;
-  const double* qb = ant8 + (int64_t)r.ib * 8;
-  double cx = cone[t], cy = cone[ntile + t], cz = cone[2 * ntile + t];
-  double ca = cone[3 * ntile + t], sa = cone[4 * ntile + t];
-  int ba = vw_cone_bits(qa[4], qa[5], qa[6], cx, cy, cz, ca, sa, qa[7]);
-  int bb = vw_cone_bits(qb[4], qb[5], qb[6], cx, cy, cz, ca, sa, qb[7]);
-  cls[k] = (uint8_t)vw_pair_class(ba, bb);
+  const RowRec r = rows[h];
+  const uint8_t* b = bits + tr * nfr_pad;
+  cls[k] = (uint8_t)vw_pair_class(b[r.ia], b[r.ib]);
 }
 
 /* effective mask of rows [row0, row0+nrows) x pixels (introspection) */
@@ -708,22 +718,15 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
 }
 
 /* class of (output tile, group) from the representative's antennas */
-__global__ void k_group_cls(const double* __restrict__ cone, int64_t ntile, int64_t tile0, int64_t nt,
+__global__ void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, int64_t nt,
                             const int32_t* __restrict__ gia, const int32_t* __restrict__ gib,
-                            const double* __restrict__ ant8, const DevInfo* __restrict__ info, int64_t nq_pad,
-                            uint8_t* __restrict__ gcls) {
+                            const DevInfo* __restrict__ info, int64_t nq_pad, uint8_t* __restrict__ gcls) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nt * nq_pad) return;
   int64_t tr = k / nq_pad, q = k - tr * nq_pad;
-  int64_t t = tile0 + tr;
   if (q >= info->nuniq) { gcls[k] = VW_CLS_HIDDEN; return; }
-  const double* qa = ant8 + (int64_t)gia[q] * 8;
-  const double* qb = ant8 + (int64_t)gib[q] * 8;
-  double cx = cone[t], cy = cone[ntile + t], cz = cone[2 * ntile + t];
-  double ca = cone[3 * ntile + t], sa = cone[4 * ntile + t];
-  int ba = vw_cone_bits(qa[4], qa[5], qa[6], cx, cy, cz, ca, sa, qa[7]);
-  int bb = vw_cone_bits(qb[4], qb[5], qb[6], cx, cy, cz, ca, sa, qb[7]);
-  gcls[k] = (uint8_t)vw_pair_class(ba, bb);
+  const uint8_t* b = bits + tr * nfr_pad;
+  gcls[k] = (uint8_t)vw_pair_class(b[gia[q]], b[gib[q]]);
 }
 
 /* ------------------------------------------------------------------ */
