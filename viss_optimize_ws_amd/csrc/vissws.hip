@@ -6,6 +6,7 @@
  * enqueue work; the public entry points synchronise before returning.
  */
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -70,6 +71,8 @@ struct Ctx {
   uint8_t* d_bits = nullptr; /* cone bits per (tile of a pass, antenna frame)            */
   int64_t nfr_pad = 0;       /* antenna frames of the longest segment, padded            */
   int64_t bits_cap = 0;      /* bytes of d_bits                                          */
+  void* d_sort_tmp = nullptr; /* radix sort scratch for segments above VW_RANK_MAX rows  */
+  size_t sort_tmp_bytes = 0;
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
   int64_t* d_counts = nullptr;
@@ -465,6 +468,14 @@ int create_device(Ctx* c) {
       if (nta > 0)
         for (int64_t w : {c->nh_pad, c->nq_pad}) maxnt = std::max(maxnt, recon_pass_tiles(c, nta, w));
     }
+    if (2 * c->max_nh > VW_RANK_MAX) {
+      HC(rocprim::radix_sort_pairs(nullptr, c->sort_tmp_bytes, (const int64_t*)nullptr, (int64_t*)nullptr,
+                                   (const int32_t*)nullptr, (int32_t*)nullptr, (unsigned)(2 * c->max_nh), 0u, 64u,
+                                   c->st));
+      uint8_t* tmp = nullptr;
+      if ((rc = dalloc(c, &tmp, c->sort_tmp_bytes))) return rc;
+      c->d_sort_tmp = tmp;
+    }
     c->nfr_pad = ((int64_t)c->max_tlen * c->A + 63) / 64 * 64;
     c->bits_cap = maxnt * c->nfr_pad;
     if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap))) return rc;
@@ -591,7 +602,14 @@ int enq_group(Ctx* c) {
   KC();
   k_keys<<<blocks_for(nr), 256, 0, c->st>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
-  {
+  if (nr > VW_RANK_MAX) {
+    /* long segments: stable LSD radix sort of (key, row index) pairs */
+    k_iota<<<blocks_for(nr), 256, 0, c->st>>>(nr, c->d_rank);
+    KC();
+    size_t tb = c->sort_tmp_bytes;
+    HC(rocprim::radix_sort_pairs(c->d_sort_tmp, tb, (const int64_t*)c->d_keys, c->d_keys_sorted,
+                                 (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->st));
+  } else {
     /* stable rank by (key, row): a grid of (256 rows) x (jblk rows) compare
      * tiles sized for about 2,048 workgroups, then a scatter */
     HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->st));

@@ -612,6 +612,14 @@ __global__ __launch_bounds__(256) void k_rank_count(const int64_t* __restrict__ 
   if (i < nr && cnt) atomicAdd(&rank[i], cnt);
 }
 
+/* segments longer than this many rows are ordered by a radix sort */
+#define VW_RANK_MAX 8192
+
+__global__ void k_iota(int64_t n, int32_t* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)i;
+}
+
 __global__ void k_rank_scatter(const int64_t* __restrict__ keys, const int32_t* __restrict__ rank, int64_t nr,
                                int32_t* __restrict__ idx_sorted, int64_t* __restrict__ keys_sorted) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
