@@ -263,7 +263,11 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   c->max_nh = (int64_t)c->max_tlen * c->P;
   c->nh_pad = (c->max_nh + 63) / 64 * 64;
   c->nq_pad = (2 * c->max_nh + 63) / 64 * 64;
-  c->generic = cfg->blockage && (cfg->recon_mode == VW_RECON_FULL || cfg->occ_mode != 2);
+  /* occ_mode 1 and 2 read one member per group: the representative kernel,
+   * with the finite-value filter of the member sets except for the uv-mirror
+   * occ_mode 2 form (recon_seg_blockage_half_rep); occ_mode 0 and 3 average
+   * several members per output point */
+  c->generic = cfg->blockage && (cfg->occ_mode == 0 || cfg->occ_mode == 3);
   if (c->max_nh > (int64_t)INT32_MAX / 4) return c->fail(VW_ERR_RANGE, "segment too long for 32-bit row indices");
   return VW_OK;
 }
@@ -625,7 +629,7 @@ int enq_group(Ctx* c) {
   k_groups<<<1, 1024, 0, c->st>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
   KC();
   k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
-                                                  c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage,
+                                                  c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode,
                                                   c->cfg.precision == VW_PREC_MIXED ? 1.0 : VW_TAB_SCALE, c->d_info,
                                                   c->gs);
   KC();
@@ -680,6 +684,7 @@ int enq_recon_set(Ctx* c, bool img) {
     a.P = c->P;
     a.full = c->cfg.recon_mode == VW_RECON_FULL;
     a.occ_mode = c->cfg.occ_mode;
+    a.rep_finite = !(c->cfg.recon_mode == VW_RECON_MIRROR && c->cfg.occ_mode == 2);
     ReconShape rs = recon_shape(np);
     if (rs.nchunk * np > c->rpart_cap) return c->fail(VW_ERR_RANGE, "internal: recon partial buffer too small");
     a.pwaves = rs.pwaves;

@@ -678,7 +678,7 @@ struct GroupSoA {
 __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* __restrict__ goff,
                              const int32_t* __restrict__ idx_sorted, const double2* __restrict__ vh,
                              const RowRec* __restrict__ rows, const double* __restrict__ ant8, int P, int full,
-                             int blockage, double uv_scale,
+                             int blockage, int occ_mode, double uv_scale,
                              const DevInfo* __restrict__ info, GroupSoA g) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= info->nuniq) return;
@@ -688,10 +688,13 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
   double ugu = 0.0, vgu = 0.0;
   if (key != 0) vw_key_uv(key, &sp, &ugu, &vgu);
   bool neg;
-  int64_t hr = vw_row_half(idx_sorted[s0 + (s1 - s0) / 2], P, full != 0, &neg);
+  /* the one member a single-representative rule reads: occ_mode 1 the
+   * first of the sorted run, occ_mode 2 the median (viss_recon_kernel.cuh:
+   * 603-614; mirror_tile_recon_group .cuh:782-792) */
+  int64_t hr = vw_row_half(idx_sorted[occ_mode == 1 ? s0 : s0 + (s1 - s0) / 2], P, full != 0, &neg);
   double vr, vi;
   if (blockage) {
-    /* representative value (mirror occ 2 uses it directly) */
+    /* representative value (signed for rows of the conjugate half) */
     double2 v = vh[hr];
     vr = v.x;
     vi = neg ? -v.y : v.y;
@@ -755,6 +758,7 @@ struct ReconArgs {
   const double* ant8;
   const DevInfo* info;
   int P, full, occ_mode;
+  int rep_finite;     /* representative kind: skip groups whose value is not finite */
   int64_t nchunk;     /* group ranges per point block                     */
   int64_t pwaves;     /* point blocks (waves) per group range             */
   int64_t pstride;    /* partial row stride                               */
@@ -927,6 +931,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
           const int cls = (int)((cw >> (8 * j)) & 0xffu);
           if (q < q0 || q >= qb_ || cls == VW_CLS_HIDDEN) continue;
           const GRec g = blk[q - q16];
+          if (KIND == VW_KIND_REP && a.rep_finite && !(isfinite(g.vr) && isfinite(g.vi))) continue;
           double vr[PPL], vi[PPL];
 #pragma unroll
           for (int k = 0; k < PPL; ++k) { vr[k] = g.vr; vi[k] = g.vi; }
