@@ -168,3 +168,27 @@ def test_tile_passes(budget, variant):
     assert np.array_equal(WIg, WIo)
     assert rel_err(Cg, Co) <= tol
     assert rel_err(Ig, Io) <= tol
+
+
+@pytest.mark.parametrize("recon_mode,precision", [(0, 0), (1, 0), (0, 1)])
+def test_uneven_segments_and_row_padding(recon_mode, precision):
+    # explicit uneven segments whose half-row counts (21 pairs x 17, 5, 31, 11
+    # epochs) are multiples of neither the rows per wave nor the 64-row padding,
+    # so the padded rows and partial row blocks are exercised
+    cfg = config(nside=32, n_ant=7, seg_tlen=[17, 5, 31, 11], img_n=64, recon_mode=recon_mode, precision=precision,
+                 freq_hz=2e7)
+    tol = MIXED_TOL if precision else PLAIN_TOL
+    chk = _seg_checks(tol, exact_groups=True) if not precision else None
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 32 * 32, seed=11), chk)
+    assert np.array_equal(Wg, Wo) and np.array_equal(WIg, WIo)
+    assert rel_err(Cg, Co) <= tol and rel_err(Ig, Io) <= tol
+
+
+def test_lunar_30mhz_band():
+    # the reference formation at its 30 MHz band (bl_max 100 km, RES ~ 46,000)
+    cfg = config(geometry=0, nside=8, freq_hz=3e7, segs=20, img_n=32)
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 8 * 8, seed=4), _seg_checks(PLAIN_TOL),
+                                                   segments={0, 13})
+    assert np.array_equal(Wg, Wo)
+    assert rel_err(Cg, Co) <= PLAIN_TOL
+    assert rel_err(Ig, Io) <= PLAIN_TOL
