@@ -272,6 +272,21 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   return VW_OK;
 }
 
+/* waves per launch the grid shapes aim for.  256 CUs hold 3,072 - 5,120
+ * waves of these kernels at a time and the work per wave varies with the
+ * culled fraction, so a launch needs many rounds of waves to keep the last
+ * round's idle tail small (measured at Nside 128: 8k -> 16k waves took
+ * Viss from 8.25 to 7.9 ms, 8k -> 32k took recon from 16.2 to 13.8 ms) */
+#ifndef VW_VISS_TARGET_WAVES
+#define VW_VISS_TARGET_WAVES 16384
+#endif
+#ifndef VW_RECON_TARGET_WAVES
+#define VW_RECON_TARGET_WAVES 32768
+#endif
+#ifndef VW_RECON_MAX_CHUNKS
+#define VW_RECON_MAX_CHUNKS 128
+#endif
+
 /* Viss launch shape for nh half rows over a pass of nt tiles */
 struct VissShape {
   int rpw;
@@ -294,7 +309,7 @@ VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {
   bool mixed = c->cfg.precision == VW_PREC_MIXED;
   s.rpw = viss_rpw(c->TP, mixed, count);
   s.nrb = (nh + s.rpw - 1) / s.rpw;
-  const int64_t target_waves = 8192;
+  const int64_t target_waves = VW_VISS_TARGET_WAVES;
   int64_t nchunk = (target_waves + s.nrb - 1) / s.nrb;
   nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, nt));
   s.tpc = (nt + nchunk - 1) / nchunk;
@@ -314,8 +329,8 @@ struct ReconShape {
 ReconShape recon_shape(int64_t npts) {
   ReconShape r;
   r.pwaves = (npts + 64 * RECON_PPL - 1) / (64 * RECON_PPL);
-  const int64_t target_waves = 8192;
-  r.nchunk = std::max<int64_t>(1, std::min<int64_t>(64, (target_waves + r.pwaves - 1) / r.pwaves));
+  const int64_t target_waves = VW_RECON_TARGET_WAVES;
+  r.nchunk = std::max<int64_t>(1, std::min<int64_t>(VW_RECON_MAX_CHUNKS, (target_waves + r.pwaves - 1) / r.pwaves));
   return r;
 }
 
