@@ -72,9 +72,6 @@ struct Ctx {
   int64_t nfr_pad = 0;       /* antenna frames of the longest segment, padded            */
   int64_t bits_cap = 0;      /* bytes of d_bits                                          */
   void* d_sort_tmp = nullptr; /* radix sort scratch for segments above VW_RANK_MAX rows  */
-  uint32_t* d_ctr = nullptr; /* work counters: Viss pair [0,1], recon pair [2,3]         */
-  uint64_t n_viss_launch = 0, n_recon_launch = 0; /* select the counter of a pair      */
-  int n_cu = 0;              /* compute units of the device                              */
   size_t sort_tmp_bytes = 0;
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
@@ -293,7 +290,7 @@ int make_plan(Ctx* c, const vw_config* cfg) {
 /* Viss launch shape for nh half rows over a pass of nt tiles */
 struct VissShape {
   int rpw;
-  int64_t tpc, nchunk, nrb;
+  int64_t tpc, nchunk, nrb, bpc, nblocks;
 };
 /* half rows per wave: as many accumulators as fit without spilling
  * (the counting variants carry one more register per row) */
@@ -319,6 +316,8 @@ VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {
   const int64_t tpc_cap = mixed ? 16 : 64; /* bounds the FP32 block length per lane */
   if (s.tpc > tpc_cap) s.tpc = tpc_cap;
   s.nchunk = (nt + s.tpc - 1) / s.tpc;
+  s.bpc = (s.nrb + 3) / 4;
+  s.nblocks = s.nchunk * s.bpc;
   return s;
 }
 
@@ -402,12 +401,6 @@ int upload_day(Ctx* c) {
 int create_device(Ctx* c) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
-  {
-    int dev = 0;
-    HC(hipGetDevice(&dev));
-    HC(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    if (c->n_cu < 1) c->n_cu = 1;
-  }
   HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   int rc;
   const size_t np = (size_t)c->npix;
@@ -506,8 +499,6 @@ int create_device(Ctx* c) {
     c->bits_cap = maxnt * c->nfr_pad;
     if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap))) return rc;
   }
-  if ((rc = dalloc(c, &c->d_ctr, 4))) return rc;
-  HC(hipMemsetAsync(c->d_ctr, 0, sizeof(uint32_t) * 4, c->st));
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
   HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * np, c->st));
@@ -559,28 +550,9 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
   return VW_OK;
 }
 
-/* workgroups of 256 threads that stay resident on the whole device */
-template <class K>
-int64_t resident_blocks(const Ctx* c, K kernel) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-  return (int64_t)per_cu * c->n_cu;
-}
-
-/* launches of the work-queue kernels: as many workgroups as stay resident
- * (never more than the items need); counters alternate within a pair */
 template <int TP, int RPW, bool MIXED, bool COUNT>
-void launch_viss(Ctx* c, VissArgs a) {
-  static int64_t resident = 0;
-  if (!resident) resident = resident_blocks(c, k_viss<TP, RPW, MIXED, COUNT>);
-  const uint64_t k = c->n_viss_launch++;
-  a.ctr = c->d_ctr + (k & 1);
-  a.ctr_next = c->d_ctr + ((k + 1) & 1);
-  const int64_t items = a.nchunk * a.nrb;
-  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(resident, (items + 3) / 4));
-  k_viss<TP, RPW, MIXED, COUNT><<<(unsigned)nb, 256, 0, c->st>>>(a);
-  /* a launch that did not run leaves the next counter uncleared */
-  if (hipPeekAtLastError() != hipSuccess) (void)hipMemsetAsync(c->d_ctr, 0, 2 * sizeof(uint32_t), c->st);
+void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
+  k_viss<TP, RPW, MIXED, COUNT><<<nb, 256, 0, st>>>(a);
 }
 
 int enq_viss(Ctx* c, bool count) {
@@ -603,10 +575,11 @@ int enq_viss(Ctx* c, bool count) {
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
     a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
     a.nh = c->nh; a.nh_pad = c->nh_pad;
-    a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb;
+    a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
     a.part = c->d_part; a.cpart = c->d_cpart;
     ev_begin(c, ST_VISSK);
-#define VW_VISS(TP_, M_, C_) launch_viss<TP_, viss_rpw(TP_, M_, C_), M_, C_>(c, a)
+    unsigned nb = (unsigned)s.nblocks;
+#define VW_VISS(TP_, M_, C_) launch_viss<TP_, viss_rpw(TP_, M_, C_), M_, C_>(a, nb, c->st)
     if (c->TP == 256) {
       if (mixed) count ? VW_VISS(256, true, true) : VW_VISS(256, true, false);
       else count ? VW_VISS(256, false, true) : VW_VISS(256, false, false);
@@ -681,16 +654,9 @@ int enq_group(Ctx* c) {
 }
 
 template <bool MIXED, int KIND>
-void launch_recon(Ctx* c, ReconArgs a) {
-  static int64_t resident = 0;
-  if (!resident) resident = resident_blocks(c, k_recon<RECON_PPL, MIXED, KIND>);
-  const uint64_t k = c->n_recon_launch++;
-  a.ctr = c->d_ctr + 2 + (k & 1);
-  a.ctr_next = c->d_ctr + 2 + ((k + 1) & 1);
-  const int64_t items = a.pwaves * a.nchunk;
-  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(resident, (items + 3) / 4));
-  k_recon<RECON_PPL, MIXED, KIND><<<(unsigned)nb, 256, 0, c->st>>>(a);
-  if (hipPeekAtLastError() != hipSuccess) (void)hipMemsetAsync(c->d_ctr + 2, 0, 2 * sizeof(uint32_t), c->st);
+void launch_recon(const ReconArgs& a, hipStream_t st) {
+  int64_t waves = a.pwaves * a.nchunk;
+  k_recon<RECON_PPL, MIXED, KIND><<<blocks_for(waves, 4), 256, 0, st>>>(a);
 }
 
 int enq_recon_set(Ctx* c, bool img) {
@@ -743,13 +709,13 @@ int enq_recon_set(Ctx* c, bool img) {
     a.wpart = c->d_rwpart;
     ev_begin(c, ST_RECONK);
     if (mixed) {
-      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(c, a);
-      else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(c, a);
-      else launch_recon<true, VW_KIND_AVG>(c, a);
+      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, c->st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, c->st);
+      else launch_recon<true, VW_KIND_AVG>(a, c->st);
     } else {
-      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(c, a);
-      else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(c, a);
-      else launch_recon<false, VW_KIND_AVG>(c, a);
+      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, c->st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, c->st);
+      else launch_recon<false, VW_KIND_AVG>(a, c->st);
     }
     KC();
     ev_end(c, ST_RECONK);

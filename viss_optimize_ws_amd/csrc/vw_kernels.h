@@ -41,6 +41,11 @@ struct DevInfo {
 
 __device__ __forceinline__ int vw_rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+/* logical block index with consecutive indices on one XCD */
+__device__ __forceinline__ int64_t vw_xcd_block(int64_t b, int64_t nb) {
+  int64_t q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
 
 __device__ __forceinline__ double vw_wave_sum(double x) {
 #pragma unroll
@@ -303,10 +308,8 @@ struct VissArgs {
   const double* ant8;
   const uint32_t* cls; /* [ntile][nh_pad] bytes (pass-relative) viewed as words */
   int64_t nh, nh_pad;
-  int64_t tpc, nchunk, nrb;
+  int64_t tpc, nchunk, nrb, bpc;
   double2* part; /* [nchunk][nh_pad] */
-  uint32_t* ctr;      /* work counter of this launch (zero at launch)       */
-  uint32_t* ctr_next; /* the next launch's counter, cleared by this one     */
   int64_t* cpart; /* [nchunk][nh_pad] (counting instance) */
 };
 
@@ -325,21 +328,6 @@ __device__ __forceinline__ void vw_tab_fill(double2* tab) {
     vw_sincos_turn((double)(i / VW_TAB_REPL) / (double)VW_TAB_N, &e[1], &e[0]);
     tab[i] = make_double2(e[0], e[1]);
   }
-}
-
-/* dynamic work distribution: the waves of a launch take work items from a
- * global counter until the items run out, so no wave idles while long items
- * finish elsewhere (a launch of fixed-size blocks leaves a partly empty last
- * round).  One lane does the vector atomic; the item is then uniform.  The
- * launch also clears the counter of the next launch of the same kernel,
- * which is unused until this launch has finished (stream order). */
-__device__ __forceinline__ uint32_t vw_next_item(uint32_t* ctr) {
-  uint32_t v = 0;
-  if ((threadIdx.x & 63) == 0) v = atomicAdd(ctr, 1u);
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(v, 0));
-}
-__device__ __forceinline__ void vw_clear_next(uint32_t* ctr_next) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr_next = 0u;
 }
 
 __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
@@ -404,15 +392,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
     vw_tab_fill(tab);
     __syncthreads();
   }
-  vw_clear_next(a.ctr_next);
-  const int lane = threadIdx.x & 63;
-  const uint32_t nitems = (uint32_t)(a.nchunk * a.nrb);
-  for (;;) {
-  /* item = chunk * nrb + row block: consecutive items share the chunk's pixels */
-  const uint32_t item = vw_next_item(a.ctr);
-  if (item >= nitems) break;
-  const int64_t chunk = item / a.nrb;
-  const int64_t rb = item - chunk * a.nrb;
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t chunk = vb / a.bpc;
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int64_t rb = (vb - chunk * a.bpc) * 4 + wave;
+  if (chunk >= a.nchunk || rb >= a.nrb) return;
   const int64_t r0 = rb * RPW; /* r0 + RPW <= nh_pad: nh_pad is a multiple of 64 */
   const int64_t t0 = a.tile0 + chunk * a.tpc;
   const int64_t tend = a.tile0 + a.ntile;
@@ -507,7 +491,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
       if (COUNT) a.cpart[chunk * a.nh_pad + h] = cc;
     }
   }
-  } /* work items */
 }
 
 /* partials -> raw half sums, chunks folded in ascending order; passes after
@@ -781,8 +764,6 @@ struct ReconArgs {
   int64_t pstride;    /* partial row stride                               */
   double* part;       /* [nchunk][pstride]                                */
   uint32_t* wpart;    /* [nchunk][pstride]                                */
-  uint32_t* ctr;      /* work counter of this launch (zero at launch)     */
-  uint32_t* ctr_next; /* the next launch's counter, cleared by this one   */
 };
 
 #define VW_KIND_REP 0
@@ -883,15 +864,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
     vw_tab_fill(tab);
     __syncthreads();
   }
-  vw_clear_next(a.ctr_next);
   const int lane = threadIdx.x & 63;
-  const uint32_t nitems = (uint32_t)(a.nchunk * a.pwaves);
-  for (;;) {
-  /* item = chunk * pwaves + point block */
-  const uint32_t item = vw_next_item(a.ctr);
-  if (item >= nitems) break;
-  const int64_t chunk = item / a.pwaves;
-  const int64_t pw = item - chunk * a.pwaves;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + vw_rfl((int)(threadIdx.x >> 6));
+  const int64_t chunk = wid / a.pwaves;
+  const int64_t pw = wid - chunk * a.pwaves;
+  if (chunk >= a.nchunk) return;
   const int64_t pbase = pw * 64 * PPL; /* pass-relative */
   const int64_t tile = pbase / a.TP;   /* pass-relative tile (class tables are per pass) */
   const vw_segparams sp = a.info->sp;
@@ -1045,7 +1022,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
     a.part[chunk * a.pstride + p] = acc.value(k);
     a.wpart[chunk * a.pstride + p] = wu + wl[k];
   }
-  } /* work items */
 }
 
 /* fold group-range partials in order into the accumulators (add_inplace) for
