@@ -227,7 +227,9 @@ def main():
     eval_per_launch = evaluated / len(per_seg)
     achieved = VISS_TERM_FLOPS_PLAIN * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
     rec_eval_step = sum(s_["rec_evaluated"] for s_ in per_seg)
-    rec_ms = tim["recon_kernel_ms"] / args.steps
+    # the HEALPix and image sets run concurrently on two streams: the recon
+    # stage's wall time (its kernels and class/reduce launches) is the basis
+    rec_ms = tim["recon_ms"] / args.steps
     rec_achieved = RECON_TERM_FLOPS_PLAIN * rec_eval_step / (rec_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -264,7 +266,7 @@ def main():
         "roofline_recon": {
             "bound": "mfma",
             "pipe": "FP64 vector ALU",
-            "kernel": "k_recon (plain, uv mirror, representative classes)",
+            "kernel": "recon stage (k_recon plain, uv mirror, representative classes; HEALPix and image sets concurrent)",
             "achieved": round(rec_achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -272,7 +274,7 @@ def main():
             "flops_per_term": RECON_TERM_FLOPS_PLAIN,
             "traffic": _traffic(args.workload, "k_recon_plain"),
             "evaluated_terms_per_step": rec_eval_step,
-            "kernel_ms_per_step": round(rec_ms, 4),
+            "stage_ms_per_step": round(rec_ms, 4),
             "fp64_instr_issue_frac": round(RECON_TERM_FP64_INSTR * rec_eval_step / (rec_ms * 1e-3) / 1e12
                                            / FP64_INSTR_PEAK_T, 4),
         },

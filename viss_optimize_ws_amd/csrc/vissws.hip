@@ -51,6 +51,8 @@ struct Ctx {
   int64_t max_nh = 0, nh_pad = 0, nchunk_max = 1;
   /* device */
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  /* lm-image recon runs beside the HEALPix recon   */
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> allocs;
   double *d_l = nullptr, *d_m = nullptr, *d_n = nullptr, *d_nm1 = nullptr, *d_cone = nullptr;
   double* d_B = nullptr;
@@ -82,6 +84,11 @@ struct Ctx {
   GroupSoA gs{};
   double* d_rpart = nullptr;
   uint32_t* d_rwpart = nullptr;
+  /* scratch of the image recon set (the HEALPix set uses d_cls, d_bits,
+   * d_rpart, d_rwpart) so the two sets can run concurrently */
+  uint8_t *d_cls2 = nullptr, *d_bits2 = nullptr;
+  double* d_rpart2 = nullptr;
+  uint32_t* d_rwpart2 = nullptr;
   int64_t nq_pad = 0;
   bool generic = false; /* recon needs per-row classes of the image tiles */
   DevInfo* d_info = nullptr;
@@ -126,7 +133,7 @@ int round_away_f(float x) { return (x >= 0.0f) ? (int)floorf(x + 0.5f) : (int)ce
 inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 /* event bracket for one stage launch group */
-hipEvent_t ev_begin(Ctx* c, Stage s) {
+hipEvent_t ev_begin(Ctx* c, Stage s, hipStream_t stream = nullptr) {
   auto& v = c->ev[s];
   if (c->ev_used[s] == v.size()) {
     EvPair p;
@@ -135,11 +142,11 @@ hipEvent_t ev_begin(Ctx* c, Stage s) {
     v.push_back(p);
   }
   EvPair& p = v[c->ev_used[s]];
-  (void)hipEventRecord(p.a, c->st);
+  (void)hipEventRecord(p.a, stream ? stream : c->st);
   return p.b;
 }
-void ev_end(Ctx* c, Stage s) {
-  (void)hipEventRecord(c->ev[s][c->ev_used[s]].b, c->st);
+void ev_end(Ctx* c, Stage s, hipStream_t stream = nullptr) {
+  (void)hipEventRecord(c->ev[s][c->ev_used[s]].b, stream ? stream : c->st);
   c->ev_used[s]++;
 }
 
@@ -402,6 +409,9 @@ int create_device(Ctx* c) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
   HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  HC(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+  HC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   int rc;
   const size_t np = (size_t)c->npix;
   if ((rc = dalloc(c, &c->d_l, np)) || (rc = dalloc(c, &c->d_m, np)) || (rc = dalloc(c, &c->d_n, np)) ||
@@ -498,6 +508,11 @@ int create_device(Ctx* c) {
     c->nfr_pad = ((int64_t)c->max_tlen * c->A + 63) / 64 * 64;
     c->bits_cap = maxnt * c->nfr_pad;
     if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap))) return rc;
+    if (c->ngp > 0 && ((rc = dalloc(c, &c->d_bits2, (size_t)c->bits_cap)) ||
+                       (rc = dalloc(c, &c->d_cls2, (size_t)c->cls_cap)) ||
+                       (rc = dalloc(c, &c->d_rpart2, (size_t)c->rpart_cap)) ||
+                       (rc = dalloc(c, &c->d_rwpart2, (size_t)c->rpart_cap))))
+      return rc;
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
@@ -534,18 +549,17 @@ int enq_gen(Ctx* c, int t0, int tlen) {
 
 /* class table of a tile pass into the class scratch: cone bits per
  * (tile, antenna frame), then per (tile, half row) or (tile, group) */
-int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups) {
+int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups,
+                uint8_t* bits, uint8_t* cls, hipStream_t st) {
   const int64_t nfr = (int64_t)c->tlen * c->A;
   if (nt * c->nfr_pad > c->bits_cap) return c->fail(VW_ERR_RANGE, "internal: cone bit buffer too small");
-  k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, c->st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad,
-                                                             c->d_bits);
+  k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad, bits);
   KC();
   if (groups)
-    k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, c->st>>>(c->d_bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib,
-                                                               c->d_info, c->nq_pad, c->d_cls);
+    k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib, c->d_info,
+                                                            c->nq_pad, cls);
   else
-    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(c->d_bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad,
-                                                              c->d_cls);
+    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, cls);
   KC();
   return VW_OK;
 }
@@ -569,7 +583,7 @@ int enq_viss(Ctx* c, bool count) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
     VissShape s = viss_shape(c, c->nh, nt, count);
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
-    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false))) return rc;
+    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits, c->d_cls, c->st))) return rc;
     VissArgs a{};
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
@@ -660,6 +674,10 @@ void launch_recon(const ReconArgs& a, hipStream_t st) {
 }
 
 int enq_recon_set(Ctx* c, bool img) {
+  /* each output set has its own stream and scratch */
+  hipStream_t st = img ? c->st2 : c->st;
+  uint8_t* cls = img ? c->d_cls2 : c->d_cls;
+  uint8_t* bits = img ? c->d_bits2 : c->d_bits;
   int kind;
   if (!c->cfg.blockage) kind = VW_KIND_AVG;
   else if (!c->generic) kind = VW_KIND_REP;
@@ -675,7 +693,7 @@ int enq_recon_set(Ctx* c, bool img) {
     const int64_t pt0 = tile0 * c->TP;
     const int64_t np = std::min(nt * c->TP, npts_all - pt0);
     if (kind != VW_KIND_AVG) {
-      int rc = enq_classes(c, cone, ntile_all, tile0, nt, kind == VW_KIND_REP);
+      int rc = enq_classes(c, cone, ntile_all, tile0, nt, kind == VW_KIND_REP, bits, cls, st);
       if (rc) return rc;
     }
     ReconArgs a{};
@@ -685,9 +703,9 @@ int enq_recon_set(Ctx* c, bool img) {
     a.pt0 = pt0;
     a.npts = np;
     a.TP = c->TP;
-    a.gcls = (const uint32_t*)c->d_cls;
+    a.gcls = (const uint32_t*)cls;
     a.nq_pad = c->nq_pad;
-    a.rcls = (const uint32_t*)c->d_cls;
+    a.rcls = (const uint32_t*)cls;
     a.nh_pad = c->nh_pad;
     a.g = c->gs;
     a.goff = c->d_goff;
@@ -705,21 +723,21 @@ int enq_recon_set(Ctx* c, bool img) {
     a.pwaves = rs.pwaves;
     a.nchunk = rs.nchunk;
     a.pstride = np;
-    a.part = c->d_rpart;
-    a.wpart = c->d_rwpart;
-    ev_begin(c, ST_RECONK);
+    a.part = img ? c->d_rpart2 : c->d_rpart;
+    a.wpart = img ? c->d_rwpart2 : c->d_rwpart;
+    ev_begin(c, ST_RECONK, st);
     if (mixed) {
-      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, c->st);
-      else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, c->st);
-      else launch_recon<true, VW_KIND_AVG>(a, c->st);
+      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, st);
+      else launch_recon<true, VW_KIND_AVG>(a, st);
     } else {
-      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, c->st);
-      else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, c->st);
-      else launch_recon<false, VW_KIND_AVG>(a, c->st);
+      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, st);
+      else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, st);
+      else launch_recon<false, VW_KIND_AVG>(a, st);
     }
     KC();
-    ev_end(c, ST_RECONK);
-    k_recon_reduce<<<blocks_for(np), 256, 0, c->st>>>(a.part, a.wpart, a.nchunk, a.pstride, pt0, np, c->cfg.blockage,
+    ev_end(c, ST_RECONK, st);
+    k_recon_reduce<<<blocks_for(np), 256, 0, st>>>(a.part, a.wpart, a.nchunk, a.pstride, pt0, np, c->cfg.blockage,
                                                       img ? c->d_Gacc : c->d_Cacc, img ? c->d_GWacc : c->d_Wacc);
     KC();
   }
@@ -730,8 +748,16 @@ int enq_recon(Ctx* c) {
   int rc;
   if ((rc = enq_group(c))) return rc;
   ev_begin(c, ST_RECON);
+  if (c->ngp) {
+    /* the image set runs on the second stream beside the HEALPix set; both
+     * read the groups and write disjoint accumulators */
+    HC(hipEventRecord(c->ev_fork, c->st));
+    HC(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    if ((rc = enq_recon_set(c, true))) return rc;
+    HC(hipEventRecord(c->ev_join, c->st2));
+  }
   if (c->cfg.healpix_out && (rc = enq_recon_set(c, false))) return rc;
-  if (c->ngp && (rc = enq_recon_set(c, true))) return rc;
+  if (c->ngp) HC(hipStreamWaitEvent(c->st, c->ev_join, 0));
   ev_end(c, ST_RECON);
   return VW_OK;
 }
@@ -831,6 +857,9 @@ VW_API void vw_destroy(void* p) {
       (void)hipEventDestroy(e.a);
       (void)hipEventDestroy(e.b);
     }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
