@@ -51,8 +51,15 @@ struct Ctx {
   int64_t max_nh = 0, nh_pad = 0, nchunk_max = 1;
   /* device */
   hipStream_t st = nullptr;
-  hipStream_t st2 = nullptr;  /* lm-image recon runs beside the HEALPix recon   */
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  /* streams: st carries the segment geometry, Viss and conjugate fill; stB
+   * the grouping and HEALPix recon; st2 the lm-image recon.  The recon of
+   * segment s thus runs beside the Viss of segment s+1 (the segment
+   * buffers alternate between two sets) */
+  hipStream_t stB = nullptr;
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_conj = nullptr;
+  hipEvent_t ev_recon[2] = {nullptr, nullptr}; /* recon done with segment buffer set b */
+  int cur = 1;                                  /* buffer set of the current segment   */
   std::vector<void*> allocs;
   double *d_l = nullptr, *d_m = nullptr, *d_n = nullptr, *d_nm1 = nullptr, *d_cone = nullptr;
   double* d_B = nullptr;
@@ -63,9 +70,17 @@ struct Ctx {
   int64_t* d_t1s = nullptr;
   double* d_circ = nullptr;
   int32_t *d_pa = nullptr, *d_pb = nullptr;
+  /* segment buffers of the current set (pointers into seg[cur]) */
   RowRec* d_rows = nullptr;
   double4* d_rowt = nullptr; /* half-row (u, v, w) * 1024 for the plain Viss phase */
   double* d_ant8 = nullptr;
+  struct SegSet {
+    RowRec* rows;
+    double4* rowt;
+    double* ant8;
+    double2 *vraw, *vh, *vfull;
+  } seg[2] = {};
+  uint8_t *d_cls_v = nullptr, *d_bits_v = nullptr; /* Viss class scratch */
   uint8_t* d_cls = nullptr; /* class scratch shared by all tile passes (cls_cap bytes) */
   int64_t cls_cap = 0;       /* bytes                                                    */
   int64_t vtpp = 0;          /* sky tiles per Viss pass                                  */
@@ -168,6 +183,8 @@ void collect_timings(Ctx* c) {
 
 int sync(Ctx* c) {
   HC(hipStreamSynchronize(c->st));
+  if (c->stB) HC(hipStreamSynchronize(c->stB));
+  if (c->st2) HC(hipStreamSynchronize(c->st2));
   collect_timings(c);
   return VW_OK;
 }
@@ -405,13 +422,34 @@ int upload_day(Ctx* c) {
   return VW_OK;
 }
 
+/* make buffer set b the current segment's */
+void select_seg(Ctx* c, int b) {
+  c->cur = b;
+  const Ctx::SegSet& g = c->seg[b];
+  c->d_rows = g.rows;
+  c->d_rowt = g.rowt;
+  c->d_ant8 = g.ant8;
+  c->d_vraw = g.vraw;
+  c->d_vh = g.vh;
+  c->d_vfull = g.vfull;
+}
+
+/* the main stream waits for every recon enqueued so far */
+int join_recon(Ctx* c) {
+  for (int b = 0; b < 2; ++b) HC(hipStreamWaitEvent(c->st, c->ev_recon[b], 0));
+  return VW_OK;
+}
+
 int create_device(Ctx* c) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
   HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   HC(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+  HC(hipStreamCreateWithFlags(&c->stB, hipStreamNonBlocking));
   HC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   HC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  HC(hipEventCreateWithFlags(&c->ev_conj, hipEventDisableTiming));
+  for (int b = 0; b < 2; ++b) HC(hipEventCreateWithFlags(&c->ev_recon[b], hipEventDisableTiming));
   int rc;
   const size_t np = (size_t)c->npix;
   if ((rc = dalloc(c, &c->d_l, np)) || (rc = dalloc(c, &c->d_m, np)) || (rc = dalloc(c, &c->d_n, np)) ||
@@ -462,10 +500,16 @@ int create_device(Ctx* c) {
   for (int k = 0; k < c->segs; ++k)
     for (int64_t nt : {c->vtpp, vlast}) nch = std::max(nch, viss_shape(c, (int64_t)c->seg_tlen[k] * c->P, nt).nchunk);
   c->nchunk_max = nch;
-  if ((rc = dalloc(c, &c->d_rows, mh)) || (rc = dalloc(c, &c->d_rowt, (size_t)c->nh_pad)) || (rc = dalloc(c, &c->d_ant8, 8 * (size_t)c->max_tlen * c->A)) ||
-      (rc = dalloc(c, &c->d_cls, (size_t)c->cls_cap)) ||
-      (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) || (rc = dalloc(c, &c->d_vraw, mh)) ||
-      (rc = dalloc(c, &c->d_vh, mh)) || (rc = dalloc(c, &c->d_vfull, 2 * mh)) ||
+  for (int b = 0; b < 2; ++b) {
+    Ctx::SegSet& g = c->seg[b];
+    if ((rc = dalloc(c, &g.rows, mh)) || (rc = dalloc(c, &g.rowt, (size_t)c->nh_pad)) ||
+        (rc = dalloc(c, &g.ant8, 8 * (size_t)c->max_tlen * c->A)) || (rc = dalloc(c, &g.vraw, mh)) ||
+        (rc = dalloc(c, &g.vh, mh)) || (rc = dalloc(c, &g.vfull, 2 * mh)))
+      return rc;
+  }
+  select_seg(c, 1);
+  if ((rc = dalloc(c, &c->d_cls, (size_t)c->cls_cap)) || (rc = dalloc(c, &c->d_cls_v, (size_t)c->cls_cap)) ||
+      (rc = dalloc(c, &c->d_part, (size_t)nch * c->nh_pad)) ||
       (rc = dalloc(c, &c->d_keys, 2 * mh)) || (rc = dalloc(c, &c->d_keys_sorted, 2 * mh)) ||
       (rc = dalloc(c, &c->d_gkeys, 2 * mh + 1)) || (rc = dalloc(c, &c->d_idx_sorted, 2 * mh)) ||
       (rc = dalloc(c, &c->d_goff, 2 * mh + 2)) || (rc = dalloc(c, &c->d_rank, 2 * mh)) ||
@@ -507,7 +551,8 @@ int create_device(Ctx* c) {
     }
     c->nfr_pad = ((int64_t)c->max_tlen * c->A + 63) / 64 * 64;
     c->bits_cap = maxnt * c->nfr_pad;
-    if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap))) return rc;
+    if ((rc = dalloc(c, &c->d_bits, (size_t)c->bits_cap)) || (rc = dalloc(c, &c->d_bits_v, (size_t)c->bits_cap)))
+      return rc;
     if (c->ngp > 0 && ((rc = dalloc(c, &c->d_bits2, (size_t)c->bits_cap)) ||
                        (rc = dalloc(c, &c->d_cls2, (size_t)c->cls_cap)) ||
                        (rc = dalloc(c, &c->d_rpart2, (size_t)c->rpart_cap)) ||
@@ -533,6 +578,9 @@ int enq_gen(Ctx* c, int t0, int tlen) {
   c->t0 = t0;
   c->tlen = tlen;
   c->nh = (int64_t)tlen * c->P;
+  /* the other buffer set: free once the recon of two segments ago is done */
+  select_seg(c, c->cur ^ 1);
+  HC(hipStreamWaitEvent(c->st, c->ev_recon[c->cur], 0));
   ev_begin(c, ST_GEN);
   int64_t na = (int64_t)tlen * c->A;
   k_seg_ant<<<blocks_for(na), 256, 0, c->st>>>(c->d_pos, c->T, c->A, t0, tlen, c->per_ant ? c->cfg.occ_radius_km : 0.0,
@@ -583,11 +631,11 @@ int enq_viss(Ctx* c, bool count) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
     VissShape s = viss_shape(c, c->nh, nt, count);
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
-    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits, c->d_cls, c->st))) return rc;
+    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
     VissArgs a{};
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
-    a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls;
+    a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls_v;
     a.nh = c->nh; a.nh_pad = c->nh_pad;
     a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.bpc = s.bpc;
     a.part = c->d_part; a.cpart = c->d_cpart;
@@ -621,6 +669,7 @@ int enq_conj(Ctx* c) {
   k_conj_fill<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_vraw, c->d_rows, c->nh, c->P, c->d_vh, c->d_vfull);
   KC();
   ev_end(c, ST_CONJ);
+  HC(hipEventRecord(c->ev_conj, c->st));
   c->have_conj = true;
   c->have_groups = false;
   return VW_OK;
@@ -630,39 +679,40 @@ int enq_group(Ctx* c) {
   if (!c->have_conj) return c->fail(VW_ERR_STATE, "vw_recon_segment needs vw_conj_fill");
   bool full = c->cfg.recon_mode == VW_RECON_FULL;
   int64_t nr = full ? 2 * c->nh : c->nh;
-  ev_begin(c, ST_GROUP);
-  k_segparams<<<1, 256, 0, c->st>>>(c->d_rows, c->nh, c->P, c->bl_max, c->lambda, c->d_info);
+  HC(hipStreamWaitEvent(c->stB, c->ev_conj, 0));
+  ev_begin(c, ST_GROUP, c->stB);
+  k_segparams<<<1, 256, 0, c->stB>>>(c->d_rows, c->nh, c->P, c->bl_max, c->lambda, c->d_info);
   KC();
-  k_keys<<<blocks_for(nr), 256, 0, c->st>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
+  k_keys<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
   if (nr > VW_RANK_MAX) {
     /* long segments: stable LSD radix sort of (key, row index) pairs */
-    k_iota<<<blocks_for(nr), 256, 0, c->st>>>(nr, c->d_rank);
+    k_iota<<<blocks_for(nr), 256, 0, c->stB>>>(nr, c->d_rank);
     KC();
     size_t tb = c->sort_tmp_bytes;
     HC(rocprim::radix_sort_pairs(c->d_sort_tmp, tb, (const int64_t*)c->d_keys, c->d_keys_sorted,
-                                 (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->st));
+                                 (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->stB));
   } else {
     /* stable rank by (key, row): a grid of (256 rows) x (jblk rows) compare
      * tiles sized for about 2,048 workgroups, then a scatter */
-    HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->st));
+    HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->stB));
     const int64_t gx = blocks_for(nr);
     const int64_t gy = std::max<int64_t>(1, std::min<int64_t>(gx, (2048 + gx - 1) / gx));
     const int64_t jblk = ((nr + gy - 1) / gy + 255) / 256 * 256;
     dim3 grid((unsigned)gx, (unsigned)((nr + jblk - 1) / jblk));
-    k_rank_count<<<grid, 256, 0, c->st>>>(c->d_keys, nr, jblk, c->d_rank);
+    k_rank_count<<<grid, 256, 0, c->stB>>>(c->d_keys, nr, jblk, c->d_rank);
     KC();
-    k_rank_scatter<<<blocks_for(nr), 256, 0, c->st>>>(c->d_keys, c->d_rank, nr, c->d_idx_sorted, c->d_keys_sorted);
+    k_rank_scatter<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_keys, c->d_rank, nr, c->d_idx_sorted, c->d_keys_sorted);
     KC();
   }
-  k_groups<<<1, 1024, 0, c->st>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
+  k_groups<<<1, 1024, 0, c->stB>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
   KC();
-  k_group_data<<<blocks_for(nr), 256, 0, c->st>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
+  k_group_data<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
                                                   c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode,
                                                   c->cfg.precision == VW_PREC_MIXED ? 1.0 : VW_TAB_SCALE, c->d_info,
                                                   c->gs);
   KC();
-  ev_end(c, ST_GROUP);
+  ev_end(c, ST_GROUP, c->stB);
   c->have_groups = true;
   return VW_OK;
 }
@@ -675,7 +725,7 @@ void launch_recon(const ReconArgs& a, hipStream_t st) {
 
 int enq_recon_set(Ctx* c, bool img) {
   /* each output set has its own stream and scratch */
-  hipStream_t st = img ? c->st2 : c->st;
+  hipStream_t st = img ? c->st2 : c->stB;
   uint8_t* cls = img ? c->d_cls2 : c->d_cls;
   uint8_t* bits = img ? c->d_bits2 : c->d_bits;
   int kind;
@@ -747,18 +797,20 @@ int enq_recon_set(Ctx* c, bool img) {
 int enq_recon(Ctx* c) {
   int rc;
   if ((rc = enq_group(c))) return rc;
-  ev_begin(c, ST_RECON);
+  ev_begin(c, ST_RECON, c->stB);
   if (c->ngp) {
-    /* the image set runs on the second stream beside the HEALPix set; both
+    /* the image set runs on its own stream beside the HEALPix set; both
      * read the groups and write disjoint accumulators */
-    HC(hipEventRecord(c->ev_fork, c->st));
+    HC(hipEventRecord(c->ev_fork, c->stB));
     HC(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
     if ((rc = enq_recon_set(c, true))) return rc;
     HC(hipEventRecord(c->ev_join, c->st2));
   }
   if (c->cfg.healpix_out && (rc = enq_recon_set(c, false))) return rc;
-  if (c->ngp) HC(hipStreamWaitEvent(c->st, c->ev_join, 0));
-  ev_end(c, ST_RECON);
+  if (c->ngp) HC(hipStreamWaitEvent(c->stB, c->ev_join, 0));
+  ev_end(c, ST_RECON, c->stB);
+  /* the segment's buffer set is free for the segment after next */
+  HC(hipEventRecord(c->ev_recon[c->cur], c->stB));
   return VW_OK;
 }
 
@@ -859,7 +911,11 @@ VW_API void vw_destroy(void* p) {
     }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_conj) (void)hipEventDestroy(c->ev_conj);
+  for (int b = 0; b < 2; ++b)
+    if (c->ev_recon[b]) (void)hipEventDestroy(c->ev_recon[b]);
   if (c->st2) (void)hipStreamDestroy(c->st2);
+  if (c->stB) (void)hipStreamDestroy(c->stB);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -995,12 +1051,14 @@ VW_API int vw_run_all(void* p) {
         (rc = enq_recon(c)))
       return rc;
   }
+  if ((rc = join_recon(c))) return rc;
   return sync(c);
 }
 
 VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* Wimg) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (int rj = join_recon(c)) return rj;
   ev_begin(c, ST_FIN);
   /* normalise on the device; copy out only what was asked for */
   if (c->cfg.healpix_out) {
@@ -1036,6 +1094,7 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
 VW_API int vw_reset_accumulators(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (int rj = join_recon(c)) return rj;
   HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
   HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * c->npix, c->st));
   if (c->ngp) {
@@ -1049,6 +1108,7 @@ VW_API int vw_reset_accumulators(void* p) {
 VW_API int vw_get_accumulators(void* p, double* C, uint32_t* W, double* G, uint32_t* GW, int64_t* wtotal) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (int rj = join_recon(c)) return rj;
   if (C) HC(hipMemcpyAsync(C, c->d_Cacc, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
   if (W) HC(hipMemcpyAsync(W, c->d_Wacc, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
   if (c->ngp && G) HC(hipMemcpyAsync(G, c->d_Gacc, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->st));
@@ -1063,6 +1123,7 @@ VW_API int vw_set_accumulators(void* p, const double* C, const uint32_t* W, cons
                                int64_t wtotal) {
   Ctx* c = as_ctx(p);
   if (!c || wtotal < 0) return VW_ERR_ARG;
+  if (int rj = join_recon(c)) return rj;
   if (C) HC(hipMemcpyAsync(c->d_Cacc, C, sizeof(double) * c->npix, hipMemcpyHostToDevice, c->st));
   else HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
   if (W) HC(hipMemcpyAsync(c->d_Wacc, W, sizeof(uint32_t) * c->npix, hipMemcpyHostToDevice, c->st));
