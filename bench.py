@@ -70,15 +70,20 @@ def _traffic(workload, kernel):
         return None
 
 
+SERIAL = False
+
+
 def _cfg(precision):
     from viss_optimize_ws_amd import default_config
 
-    return default_config(1, precision=precision, **WORKLOAD)
+    return default_config(1, precision=precision, serial_stages=int(SERIAL), **WORKLOAD)
 
 
 def _plan_counts(pipe):
     """One instrumented pass over the schedule: per-segment nkeys, evaluated
-    (non-culled) Viss terms, and the final maps for error statements."""
+    (non-culled) Viss terms, the final maps for error statements, and the
+    stage timings of the pass -- each stage call synchronises, so every
+    kernel runs alone (the basis of the roofline figures)."""
     per_seg = []
     TP = pipe.cfg.tile_pix
 
@@ -90,6 +95,7 @@ def _plan_counts(pipe):
     tile_sizes = sizes(pipe.plan.npix, pipe.plan.ntile)
     img_sizes = sizes(pipe.plan.n_img_pts, pipe.plan.n_img_tiles) if pipe.plan.n_img_pts else None
     pipe.reset_accumulators()
+    pipe.timings()  # clear
     for t0, tl in pipe.schedule():
         pipe.gen_segment(t0, tl)
         cls = pipe.tile_classes()
@@ -108,7 +114,7 @@ def _plan_counts(pipe):
         per_seg.append(dict(n_half=info["n_half"], nkeys=info["nkeys"], evaluated=evaluated, rec_evaluated=rec_eval,
                             vraw=vh))
     maps = pipe.finalize()
-    return per_seg, maps
+    return per_seg, maps, pipe.timings()
 
 
 def _time_steps(pipe, steps, warmup):
@@ -192,6 +198,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--serial", action="store_true",
+                    help="run every stage on one stream (no overlap of a segment's recon with the next Viss)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nside128",
                     help="nside128 (default, the headline configuration) or nside256")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -200,8 +208,9 @@ def main():
     if args.copy_calib:
         _copy_calibration(args.copy_calib)
         return
-    global WORKLOAD, WORKLOAD_NAME
+    global WORKLOAD, WORKLOAD_NAME, SERIAL
     WORKLOAD, WORKLOAD_NAME = WORKLOADS[args.workload]
+    SERIAL = args.serial
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one MI355X)",
               file=sys.stderr)
@@ -214,7 +223,7 @@ def main():
         npix, nimg = plain.plan.npix, plain.plan.n_img_pts
         sky = make_sky(sky_seed, npix, 0)
         plain.set_sky(sky)
-        per_seg, maps_plain = _plan_counts(plain)
+        per_seg, maps_plain, tim_iso = _plan_counts(plain)
         t_viss = sum(npix * s["n_half"] for s in per_seg)
         t_recon = sum((npix + nimg) * s["nkeys"] for s in per_seg)
         terms = t_viss + t_recon
@@ -222,14 +231,15 @@ def main():
         sec, tim = _time_steps(plain, args.steps, args.warmup)
 
     value = terms / sec / 1e9
-    nl = max(tim["n_viss_launch"], 1)
-    viss_avg_ms = tim["viss_kernel_ms"] / nl
+    nl = max(tim_iso["n_viss_launch"], 1)
+    viss_avg_ms = tim_iso["viss_kernel_ms"] / nl
     eval_per_launch = evaluated / len(per_seg)
     achieved = VISS_TERM_FLOPS_PLAIN * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
     rec_eval_step = sum(s_["rec_evaluated"] for s_ in per_seg)
-    # the HEALPix and image sets run concurrently on two streams: the recon
-    # stage's wall time (its kernels and class/reduce launches) is the basis
-    rec_ms = tim["recon_ms"] / args.steps
+    # recon stage of the instrumented pass (the HEALPix and image sets run
+    # concurrently on two streams): its wall time, kernels and class/reduce
+    # launches, per step
+    rec_ms = tim_iso["recon_ms"]
     rec_achieved = RECON_TERM_FLOPS_PLAIN * rec_eval_step / (rec_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -245,6 +255,7 @@ def main():
         "dtype": "f64",
         "data": f"synthetic: log-normal(0,1) sky seed {sky_seed}, circular orbits seed 42",
         "config": {"workload": WORKLOAD_NAME, **WORKLOAD, "form": "plain FP64",
+                   "stages": "serial" if SERIAL else "recon of segment s beside the Viss of segment s+1",
                    "terms_per_step": terms, "viss_terms": t_viss, "recon_terms": t_recon},
         "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in tim.items() if k.endswith("_ms")},
         "roofline": {
@@ -260,6 +271,7 @@ def main():
             "flops_per_term": VISS_TERM_FLOPS_PLAIN,
             "evaluated_terms_per_launch": int(eval_per_launch),
             "avg_launch_ms": round(viss_avg_ms, 4),
+            "timing": "HIP events around each k_viss launch of the instrumented pass (stages run alone)",
             "fp64_instr_issue_frac": round(VISS_TERM_FP64_INSTR * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
                                            / FP64_INSTR_PEAK_T, 4),
         },
@@ -282,7 +294,7 @@ def main():
     if not args.no_mixed:
         with Pipeline(_cfg(1)) as mixed:
             mixed.set_sky(sky)
-            per_seg_m, maps_m = _plan_counts(mixed)
+            per_seg_m, maps_m, tim_m_iso = _plan_counts(mixed)
             sec_m, tim_m = _time_steps(mixed, args.steps, args.warmup)
         from_err = {}
         vr_p = np.concatenate([s["vraw"] for s in per_seg])
@@ -295,7 +307,7 @@ def main():
         out["mixed"] = {"value": round(terms / sec_m / 1e9, 4), "unit": "Gterms/s", "ms_per_step": round(sec_m * 1e3, 4),
                         "speedup_vs_plain": round(sec / sec_m, 3), "dtype": "f64 phase + f32 sincos/sum",
                         "err_vs_fp64": from_err,
-                        "viss_kernel_avg_ms": round(tim_m["viss_kernel_ms"] / max(tim_m["n_viss_launch"], 1), 4)}
+                        "viss_kernel_avg_ms": round(tim_m_iso["viss_kernel_ms"] / max(tim_m_iso["n_viss_launch"], 1), 4)}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = _cpu_baseline(sky, threads, per_seg[0]["vraw"], args.workload == "nside128")

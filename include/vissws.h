@@ -73,6 +73,9 @@ typedef struct {
   int64_t class_budget_bytes; /* device bytes for the per-(tile, row) class tables of
                                * one tile pass (0 = 2 GiB); a smaller budget splits the
                                * Viss and recon stages into more tile passes      */
+  int32_t serial_stages;       /* 1: run every stage on one stream (no overlap of the
+                                * recon of a segment with the next segment's Viss;
+                                * for profiling single kernels)                     */
 } vw_config;
 
 /* Derived plan (filled by vw_get_plan). */

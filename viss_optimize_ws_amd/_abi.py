@@ -51,6 +51,7 @@ class vw_config(C.Structure):
         ("r_max_km", C.c_double),
         ("gm_km3_s2", C.c_double),
         ("class_budget_bytes", C.c_int64),
+        ("serial_stages", C.c_int32),
     ]
 
 

@@ -183,8 +183,8 @@ void collect_timings(Ctx* c) {
 
 int sync(Ctx* c) {
   HC(hipStreamSynchronize(c->st));
-  if (c->stB) HC(hipStreamSynchronize(c->stB));
-  if (c->st2) HC(hipStreamSynchronize(c->st2));
+  if (c->stB && c->stB != c->st) HC(hipStreamSynchronize(c->stB));
+  if (c->st2 && c->st2 != c->st) HC(hipStreamSynchronize(c->st2));
   collect_timings(c);
   return VW_OK;
 }
@@ -444,8 +444,12 @@ int create_device(Ctx* c) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
   HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-  HC(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
-  HC(hipStreamCreateWithFlags(&c->stB, hipStreamNonBlocking));
+  if (c->cfg.serial_stages) {
+    c->st2 = c->stB = c->st; /* one stream: the stream waits below are no-ops */
+  } else {
+    HC(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    HC(hipStreamCreateWithFlags(&c->stB, hipStreamNonBlocking));
+  }
   HC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   HC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   HC(hipEventCreateWithFlags(&c->ev_conj, hipEventDisableTiming));
@@ -914,8 +918,8 @@ VW_API void vw_destroy(void* p) {
   if (c->ev_conj) (void)hipEventDestroy(c->ev_conj);
   for (int b = 0; b < 2; ++b)
     if (c->ev_recon[b]) (void)hipEventDestroy(c->ev_recon[b]);
-  if (c->st2) (void)hipStreamDestroy(c->st2);
-  if (c->stB) (void)hipStreamDestroy(c->stB);
+  if (c->st2 && c->st2 != c->st) (void)hipStreamDestroy(c->st2);
+  if (c->stB && c->stB != c->st) (void)hipStreamDestroy(c->stB);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
