@@ -79,11 +79,16 @@ def _cfg(precision):
     return default_config(1, precision=precision, serial_stages=int(SERIAL), **WORKLOAD)
 
 
+def _cfg_serial(precision):
+    from viss_optimize_ws_amd import default_config
+
+    return default_config(1, precision=precision, serial_stages=1, **WORKLOAD)
+
+
 def _plan_counts(pipe):
     """One instrumented pass over the schedule: per-segment nkeys, evaluated
     (non-culled) Viss terms, the final maps for error statements, and the
-    stage timings of the pass -- each stage call synchronises, so every
-    kernel runs alone (the basis of the roofline figures)."""
+    stage timings of the pass."""
     per_seg = []
     TP = pipe.cfg.tile_pix
 
@@ -229,17 +234,25 @@ def main():
         terms = t_viss + t_recon
         evaluated = sum(s["evaluated"] for s in per_seg)
         sec, tim = _time_steps(plain, args.steps, args.warmup)
+    if SERIAL:
+        tim_ser = tim
+    else:
+        # kernel-level figures from the same steps with every stage on one
+        # stream, where each kernel runs alone
+        with Pipeline(_cfg_serial(0)) as ser:
+            ser.set_sky(sky)
+            _, tim_ser = _time_steps(ser, args.steps, args.warmup)
 
     value = terms / sec / 1e9
-    nl = max(tim_iso["n_viss_launch"], 1)
-    viss_avg_ms = tim_iso["viss_kernel_ms"] / nl
+    nl = max(tim_ser["n_viss_launch"], 1)
+    viss_avg_ms = tim_ser["viss_kernel_ms"] / nl
     eval_per_launch = evaluated / len(per_seg)
     achieved = VISS_TERM_FLOPS_PLAIN * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
     rec_eval_step = sum(s_["rec_evaluated"] for s_ in per_seg)
-    # recon stage of the instrumented pass (the HEALPix and image sets run
+    # recon stage of the serial steps (the HEALPix and image sets run
     # concurrently on two streams): its wall time, kernels and class/reduce
     # launches, per step
-    rec_ms = tim_iso["recon_ms"]
+    rec_ms = tim_ser["recon_ms"] / args.steps
     rec_achieved = RECON_TERM_FLOPS_PLAIN * rec_eval_step / (rec_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -271,7 +284,7 @@ def main():
             "flops_per_term": VISS_TERM_FLOPS_PLAIN,
             "evaluated_terms_per_launch": int(eval_per_launch),
             "avg_launch_ms": round(viss_avg_ms, 4),
-            "timing": "HIP events around each k_viss launch of the instrumented pass (stages run alone)",
+            "timing": "HIP events around each k_viss launch of the timed steps run with serial stages (kernel alone)",
             "fp64_instr_issue_frac": round(VISS_TERM_FP64_INSTR * eval_per_launch / (viss_avg_ms * 1e-3) / 1e12
                                            / FP64_INSTR_PEAK_T, 4),
         },
