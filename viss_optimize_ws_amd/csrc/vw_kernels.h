@@ -335,6 +335,18 @@ __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
   *c = __builtin_amdgcn_cosf(r);
 }
 
+/* occlusion of the PPL pixels of a lane against both endpoints of a row: the
+ * plain kernels evaluate both tests without short circuit (straight-line
+ * code, no exec-mask branches); the mixed kernels, at their tighter register
+ * budget, keep the short-circuit form */
+template <bool BRANCHFREE>
+__device__ __forceinline__ bool vw_vis2(bool ok, double l, double m, double n, double ax, double ay, double az,
+                                        double ai, double ac, double bx, double by, double bz, double bi, double bc) {
+  if (BRANCHFREE)
+    return ((int)ok & vw_visible(l, m, n, ax, ay, az, ai, ac) & vw_visible(l, m, n, bx, by, bz, bi, bc)) != 0;
+  return ok && vw_visible(l, m, n, ax, ay, az, ai, ac) && vw_visible(l, m, n, bx, by, bz, bi, bc);
+}
+
 /* mixed form, one term: accumulate bf * exp(-j 2 pi x) in FP32 (the sign of
  * the imaginary part is applied when the partial is written) */
 __device__ __forceinline__ void vw_viss_term_mixed(const RowRec& rec, double l, double m, double nm1, float bf,
@@ -457,8 +469,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
         bool vis[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-          vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                   vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+          vis[k] = MIXED ? (ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                            vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc))
+                         : vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
         if (MIXED) {
 #pragma unroll
           for (int k = 0; k < PPL; ++k)
@@ -771,7 +784,7 @@ struct ReconArgs {
 #define VW_KIND_AVG 2
 
 /* visibility of one row for the PPL points of a lane given the class */
-template <int PPL>
+template <int PPL, bool BRANCHFREE>
 __device__ __forceinline__ void vw_ant_vis(const double* __restrict__ ant8, int ia, int ib, const double* pl,
                                            const double* pm, const double* pn, const bool* ok, bool* vis) {
   const double* qa = ant8 + (int64_t)ia * 8;
@@ -780,8 +793,8 @@ __device__ __forceinline__ void vw_ant_vis(const double* __restrict__ ant8, int 
   const double bx = qb[0], by = qb[1], bz = qb[2], bi = qb[3], bc = qb[7];
 #pragma unroll
   for (int k = 0; k < PPL; ++k)
-    vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-             vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+    vis[k] = vw_vis2<BRANCHFREE>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz,
+             bi, bc);
 }
 
 /* FP64 phase and turn reduction, hardware FP32 sin/cos, FP32 block sums
@@ -856,8 +869,8 @@ struct ReconLane {
 #define VW_RECON_WAVES_MIXED 5
 #endif
 template <int PPL, bool MIXED, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_RECON_WAVES_MIXED
-                                                                            : VW_RECON_WAVES_PLAIN))) void k_recon(ReconArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    MIXED && KIND != VW_KIND_GENERIC ? VW_RECON_WAVES_MIXED : VW_RECON_WAVES_PLAIN))) void k_recon(ReconArgs a) {
   __shared__ double2 tab[MIXED ? 1 : VW_TAB_N * VW_TAB_REPL];
   __shared__ double gring[4][2 * 64]; /* per wave: two blocks of 16 group records */
   if (!MIXED) {
@@ -948,8 +961,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
             bool vis[PPL];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-              vis[k] = ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                       vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc);
+              vis[k] = MIXED ? (ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
+                                vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc))
+                             : vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
               wl[k] += vis[k] ? 1 : 0;
             }
             acc.add(g.ugu, g.vgu, vr, vi, vis, lp, mp, tab);
@@ -988,7 +1002,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
           for (int k = 0; k < PPL; ++k) mv[k] = ok[k];
         } else {
           const RowRec rec = a.rows[h];
-          vw_ant_vis<PPL>(a.ant8, rec.ia, rec.ib, pl, pm, pn, ok, mv);
+          vw_ant_vis<PPL, !MIXED>(a.ant8, rec.ia, rec.ib, pl, pm, pn, ok, mv);
         }
         const double2 v = a.vh[h];
         const double wr = v.x, wi = ng ? -v.y : v.y;
