@@ -313,19 +313,12 @@ struct VissArgs {
   int64_t* cpart; /* [nchunk][nh_pad] (counting instance) */
 };
 
-/* LDS sine table: VW_TAB_REPL copies interleaved entry by entry, lane l
- * reading copy l mod VW_TAB_REPL, so that lanes served in one LDS pass hit
- * different banks whatever their indices */
-#ifndef VW_TAB_REPL
-#define VW_TAB_REPL 1
-#endif
-__device__ __forceinline__ int vw_tab_slot(int idx) {
-  return idx * VW_TAB_REPL + (int)(threadIdx.x & (VW_TAB_REPL - 1));
-}
+/* the LDS sine table, built by every workgroup of the plain kernels:
+ * entry k = (cos, sin)(2 pi k / N), as vw_sincos_tab_build */
 __device__ __forceinline__ void vw_tab_fill(double2* tab) {
-  for (int i = threadIdx.x; i < VW_TAB_N * VW_TAB_REPL; i += blockDim.x) {
+  for (int i = threadIdx.x; i < VW_TAB_N; i += blockDim.x) {
     double e[2]; /* (cos, sin) as vw_sincos_tab_build */
-    vw_sincos_turn((double)(i / VW_TAB_REPL) / (double)VW_TAB_N, &e[1], &e[0]);
+    vw_sincos_turn((double)i / (double)VW_TAB_N, &e[1], &e[0]);
     tab[i] = make_double2(e[0], e[1]);
   }
 }
@@ -372,7 +365,7 @@ __device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const dou
   for (int k = 0; k < PPL; ++k) f[k] = vw_tab_split(vw_viss_phase(rt.x, rt.y, rt.z, pl[k], pm[k], pz[k]), &ix[k]);
   double2 e[PPL];
 #pragma unroll
-  for (int k = 0; k < PPL; ++k) e[k] = tab[vw_tab_slot(ix[k])];
+  for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     double sy, cy, sn, cs;
@@ -399,7 +392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
   constexpr int PPL = 4;
   constexpr int NSUB = TP / 256;
   static_assert(RPW % 4 == 0, "class words hold 4 rows");
-  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N * VW_TAB_REPL];
+  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
   if (!MIXED) {
     vw_tab_fill(tab);
     __syncthreads();
@@ -842,7 +835,7 @@ struct ReconLane {
       for (int k = 0; k < PPL; ++k) fr[k] = vw_tab_split(vw_recon_phase(ugu, vgu, lp[k], mp[k]), &ix[k]);
       double2 e[PPL];
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) e[k] = tab[vw_tab_slot(ix[k])];
+      for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
         double sy, cy, sn, cs;
@@ -871,7 +864,7 @@ struct ReconLane {
 template <int PPL, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     MIXED && KIND != VW_KIND_GENERIC ? VW_RECON_WAVES_MIXED : VW_RECON_WAVES_PLAIN))) void k_recon(ReconArgs a) {
-  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N * VW_TAB_REPL];
+  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
   __shared__ double gring[4][2 * 64]; /* per wave: two blocks of 16 group records */
   if (!MIXED) {
     vw_tab_fill(tab);
