@@ -1,6 +1,7 @@
 /*
  * vissws.hip -- host orchestration and C-ABI of the gfx950 path
- * (include/vissws.h).  One HIP stream per context at default priority; all
+ * (include/vissws.h).  Three HIP streams per context, all at default
+ * priority (geometry + Viss; grouping + HEALPix recon; lm-image recon); all
  * device memory is allocated once in vw_create and sized for the longest
  * segment, so the segment loop performs no allocation.  Stage functions
  * enqueue work; the public entry points synchronise before returning.

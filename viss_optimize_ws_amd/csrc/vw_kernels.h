@@ -3,15 +3,18 @@
  *
  * Execution model (wave64, CDNA4):
  *  - Viss: a wave owns one chunk of sky tiles x RPW half rows.  Lanes hold
- *    pixels (TP/64 per lane, coalesced SoA loads), rows are wave-uniform:
- *    row records, tile-cone classes and antenna frames arrive through scalar
- *    loads, so the all-hidden / all-visible / mixed decision is a scalar
- *    branch with no divergence.  Each lane keeps RPW complex partial sums in
- *    registers; one cross-lane tree per row per chunk writes a partial that
- *    a second kernel folds in chunk order (deterministic).
- *  - Recon: lanes hold output points (one tile per wave group), groups are
- *    wave-uniform; the plain form sums groups in ascending order per point,
- *    exactly as the oracle does.
+ *    pixels (4 per lane per 256-pixel sub-tile, coalesced SoA loads), rows
+ *    are wave-uniform: row records, tile-cone classes and antenna frames
+ *    arrive through scalar loads, so the all-hidden / all-visible / mixed
+ *    decision is a scalar branch with no divergence.  Each lane keeps RPW
+ *    complex partial sums in registers; one cross-lane tree per row per chunk
+ *    writes a partial that a second kernel folds in chunk order
+ *    (deterministic).  The plain form evaluates sin/cos from a 1024-entry
+ *    LDS table (vw_shared.h).
+ *  - Recon: lanes hold output points (4 per lane, one output tile per wave),
+ *    groups are wave-uniform and pass through a per-wave LDS ring; the group
+ *    range of a segment is split into chunks whose partials a second kernel
+ *    folds in order.
  * Workgroups are remapped so that consecutive logical blocks land on the same
  * XCD (hardware dispatches block b to XCD b % 8), keeping a chunk's pixel
  * data in one XCD's L2.
