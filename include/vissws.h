@@ -41,6 +41,17 @@ extern "C" {
 #define VW_RECON_MIRROR 0 /* half uv grid  (recon_mirror/, mirror_tile_*) */
 #define VW_RECON_FULL 1   /* full uv grid  (main_recon_online.cu)         */
 
+/* Viss summation form (vw_config.viss_form) */
+#define VW_VISS_AUTO 0      /* per-row sums in the plain form; antenna-factorized in the mixed form
+                             * for n_ant <= 16                                                   */
+#define VW_VISS_ROWS 1      /* one phase / sin / cos term per (half row, pixel): the reference's
+                             * phase rounding, bit for bit                                       */
+#define VW_VISS_ANT 2       /* antenna-factorized on the matrix cores in either form (n_ant <= 32);
+                             * in FP64 it differs from the per-row sums by the rounding of the
+                             * phase, about 2^-52 |u| turns per term                             */
+#define VW_VISS_ANT_CHECK 3 /* as VW_VISS_ANT, with the correction pass run on every tile that has a
+                             * partly visible antenna (exercises that path; slower)              */
+
 typedef struct {
   double re, im;
 } vw_c128;
@@ -76,6 +87,8 @@ typedef struct {
   int32_t serial_stages;       /* 1: run every stage on one stream (no overlap of the
                                 * recon of a segment with the next segment's Viss;
                                 * for profiling single kernels)                     */
+  int32_t viss_form;           /* VW_VISS_*: summation form of the Viss stage (the
+                                * half visibilities agree to FP64 rounding)         */
 } vw_config;
 
 /* Derived plan (filled by vw_get_plan). */

@@ -192,3 +192,42 @@ def test_lunar_30mhz_band():
     assert np.array_equal(Wg, Wo)
     assert rel_err(Cg, Co) <= PLAIN_TOL
     assert rel_err(Ig, Io) <= PLAIN_TOL
+
+
+# the antenna-factorized FP64 form sums exp(-2 pi j (psi_b - psi_a)) with
+# per-antenna phases psi, so its phases differ from the per-row phase
+# u l + v m + w (n-1) by FP64 rounding, about 2^-52 |u| turns per term (|u| up
+# to ~1e6 wavelengths here): a looser bound than the per-row form's 1e-9
+ANT_PLAIN_TOL = 1e-7
+
+
+@pytest.mark.parametrize(
+    "n_ant,form,precision",
+    [(8, 1, 0), (8, 3, 0), (13, 2, 0), (16, 0, 1), (24, 2, 0), (24, 3, 1), (32, 2, 1), (32, 0, 1), (40, 0, 1)],
+    ids=lambda v: str(v),
+)
+def test_viss_forms(n_ant, form, precision):
+    # the per-row form (viss_form 1), the antenna-factorized form (2; the mixed
+    # form's default, 0, up to 16 antennas) with one (n_ant <= 16) or two
+    # (17..32) groups of 16 antennas, its correction pass on every tile with a
+    # partly visible antenna (3), and the per-row form the mixed default takes
+    # above 16 antennas
+    cfg = config(nside=32, n_ant=n_ant, n_epochs=12, segs=2, img_n=0, viss_form=form, precision=precision,
+                 freq_hz=2e7)
+    tol = MIXED_TOL if precision else (PLAIN_TOL if form == 1 else ANT_PLAIN_TOL)
+    sky = _sky(12 * 32 * 32, seed=n_ant)
+    errs = []
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        g.set_sky(sky)
+        o.set_sky(sky)
+        for t0, tl in g.schedule():
+            g.gen_segment(t0, tl)
+            o.gen_segment(t0, tl)
+            vg = g.viss_half()
+            vo = o.viss_half()
+            errs.append(rel_err(vg, vo))
+            assert errs[-1] <= tol, (t0, errs[-1])
+            hg, _ = g.conj_fill()
+            ho, _ = o.conj_fill()
+            assert rel_err(hg, ho) <= tol
+    print(f"viss_form {form} n_ant {n_ant} precision {precision}: max rel err {max(errs):.3e}")

@@ -20,6 +20,10 @@ VW_PREC_PLAIN = 0
 VW_PREC_MIXED = 1
 VW_RECON_MIRROR = 0
 VW_RECON_FULL = 1
+VW_VISS_AUTO = 0
+VW_VISS_ROWS = 1
+VW_VISS_ANT = 2
+VW_VISS_ANT_CHECK = 3
 
 CLS_MIXED, CLS_VISIBLE, CLS_HIDDEN = 0, 1, 2
 
@@ -52,6 +56,7 @@ class vw_config(C.Structure):
         ("gm_km3_s2", C.c_double),
         ("class_budget_bytes", C.c_int64),
         ("serial_stages", C.c_int32),
+        ("viss_form", C.c_int32),
     ]
 
 

@@ -15,6 +15,7 @@
  *   --precision={plain,mixed} --recon={full,mirror} --img_n --bl_max_m
  *   --sky={file,lognormal,lognormal_ps} --sky_seed --occ_radius_km
  *   --occ_altitude_km
+ *   --viss_form={auto,rows,ant}  Viss summation form (vw_config.viss_form)
  *   --checkpoint=PATH  segment-level checkpoint (raw accumulators, position
  *                      in the schedule, text-output sizes) after every
  *                      segment and day; --resume=1 continues from it;
@@ -172,6 +173,7 @@ int main(int argc, char** argv) {
   cfg.tile_pix = to_int(get_arg(argc, argv, "--viss_tile_pix", "256"), 256);
   std::string prec = get_arg(argc, argv, "--precision", "plain");
   std::string recon = get_arg(argc, argv, "--recon", "full");
+  std::string vform = get_arg(argc, argv, "--viss_form", "auto");
   cfg.img_n = to_int(get_arg(argc, argv, "--img_n", "0"), 0);
   cfg.bl_max_m = to_dbl(get_arg(argc, argv, "--bl_max_m", "0"), 0.0);
   std::string sky_kind = get_arg(argc, argv, "--sky", "file");
@@ -213,6 +215,11 @@ int main(int argc, char** argv) {
     return 1;
   }
   cfg.recon_mode = recon == "mirror" ? VW_RECON_MIRROR : VW_RECON_FULL;
+  if (vform != "auto" && vform != "rows" && vform != "ant") {
+    std::fprintf(stderr, "ERROR --viss_form\n");
+    return 1;
+  }
+  cfg.viss_form = vform == "rows" ? VW_VISS_ROWS : (vform == "ant" ? VW_VISS_ANT : VW_VISS_AUTO);
   if (C_mode != "txt" && C_mode != "bin") {
     std::fprintf(stderr, "ERROR --C_mode\n");
     return 1;
@@ -276,6 +283,7 @@ int main(int argc, char** argv) {
   fp.v(cfg.nside); fp.v(cfg.freq_hz); fp.v((int32_t)geometry); fp.v(cfg.n_ant); fp.v(plan.n_epochs);
   fp.v(cfg.dt_s); fp.v(cfg.orbit_seed); fp.v(plan.segs); fp.v(cfg.blockage); fp.v(cfg.occ_mode);
   fp.v(cfg.recon_mode); fp.v(cfg.precision); fp.v(cfg.tile_pix); fp.v(cfg.img_n); fp.v(plan.bl_max_m);
+  fp.v(cfg.viss_form);
   fp.v(cfg.occ_radius_km); fp.v(cfg.occ_altitude_km); fp.v(cfg.r_min_km); fp.v(cfg.r_max_km);
   fp.str(btag); fp.str(sky_kind); fp.v(sky_seed); fp.str(sky_kind == "file" ? sky_dir : std::string());
   fp.v(write_viss); fp.v(write_baseline_txt);

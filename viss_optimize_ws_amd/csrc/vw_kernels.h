@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "vw_shared.h"
 #include "../../include/vissws.h"
 
@@ -525,6 +526,343 @@ __global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, 
   }
   vraw[h] = make_double2(re, im);
   if (counts) counts[h] = c;
+}
+
+/* ------------------------------------------------------------------ */
+/* Viss, antenna-factorized form on the matrix cores                    */
+/* ------------------------------------------------------------------ */
+/* The phase of half row (a, b) is a difference of antenna phases,
+ *   u l + v m + w (n-1) = psi_b - psi_a,  psi_x = (x - x_0).(l, m, n-1) 1e3/lambda
+ * (x_0: the epoch's antenna 0, which cancels), and its mask is a product of
+ * per-antenna masks (below).  With e_x = exp(+2 pi j psi_x) = c_x + j s_x one
+ * epoch's half visibilities are entries of
+ *   V[a][b] = sum_p I_p m_a m_b e_a conj(e_b):
+ *   Re V = Y11[a][b] + Y22[a][b],  Im V = Y21[a][b] - Y21[b][a],
+ *   Y11 = sum I c c^T,  Y22 = sum I s s^T,  Y21 = sum I s c^T,
+ * three real (16 x 4) x (4 x 16) products per 4 pixels and 16 antennas on
+ * the FP64 matrix cores (FP32 in the mixed form), where the row form
+ * evaluates 120 phase / sin / cos terms per pixel one by one.  Lane l
+ * computes the exponential of antenna 16 gi + l % 16 at pixel l / 16 of the
+ * step, which is at once its A operand (row, k) and its B operand (k, column)
+ * element.  Accumulator layout (measured on gfx950): FP64 lane l register r
+ * holds D[l/16 + 4r][l%16]; FP32 D[4(l/16) + r][l%16].
+ *
+ * Masks.  The mask of row (a, b) on a pixel is 1 when both endpoint cones
+ * hold the tile (class visible), 0 when either cone misses it (hidden), else
+ * the product of the two per-pixel predicates.  In a tile where no antenna
+ * is partly visible the per-antenna mask m = hidden ? 0 : 1 gives exactly
+ * that; otherwise m = hidden ? 0 : predicate does, unless the predicate of an
+ * all-visible antenna fails on a pixel of the tile (the cone bound is
+ * conservative, so only rounding at the horizon can do that), which would
+ * drop terms of pairs of two all-visible antennas.  The summing pass flags
+ * that case; the tile then gets a correction pass adding, for those pairs,
+ *   Y += sum_p I_p (o_a o_b - o_a p_a o_b p_b) (...)   (o: all-visible),
+ * two more products per block on the same accumulators. */
+typedef double vw_d4 __attribute__((ext_vector_type(4)));
+typedef float vw_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ vw_d4 vw_mfma(double a, double b, vw_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ vw_f4 vw_mfma(float a, float b, vw_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct VissFArgs {
+  const double *l, *m, *n, *nm1, *B;
+  const float* Bf;
+  int64_t npix;
+  int64_t tile0, ntile;   /* tiles of this pass: [tile0, tile0 + ntile) */
+  const double4* aph;     /* [tlen*A] (x - x_0) * 1e3 / lambda * 1024 per antenna frame */
+  const double* ant8;     /* [tlen*A][8] antenna frames */
+  const uint8_t* bits;    /* [ntile][nfr_pad] cone bits, pass-relative tiles */
+  int64_t nfr_pad;
+  int32_t A, tlen;
+  int64_t tpc, nchunk, nquad;
+  int32_t check;          /* 1: every tile with a partly visible antenna takes the two-mask sum */
+  double* part;           /* [tlen][nchunk][NB][256] */
+};
+
+/* blocks of one wave's accumulators for G groups of 16 antennas: Y11 and Y22
+ * for gi >= gj (symmetric), Y21 for all (gi, gj) */
+template <int G> struct VfShape {
+  static constexpr int NS = G * (G + 1) / 2;
+  static constexpr int NB = NS + G * G; /* partial blocks: Re = Y11 + Y22 (NS), Y21 (G*G) */
+};
+
+/* one tile's pixel data staged in LDS for the four waves of a workgroup
+ * (weights zero past the end of the sky) */
+template <int TP> struct VfTileLds {
+  double l[TP], m[TP], n[TP], nm1[TP], w[TP];
+};
+
+template <int G> struct VfLane {
+  double kx[G], ky[G], kz[G];                 /* phase coefficients, 1/1024 turns */
+  double px[G], py[G], pz[G], pinv[G], pcos[G]; /* predicate frame */
+};
+
+/* the product-mask sum of one tile.  PRED false (no partly visible
+ * antenna in the tile): m = hidden ? 0 : 1.  PRED: m = hidden ? 0 :
+ * predicate, and the return value holds the lanes whose predicate failed on
+ * some pixel (a lane mask; only all-visible antennas' lanes matter).  CORR:
+ * the correction pass instead, for pairs of two all-visible antennas
+ * (o = all-visible): Y += sum w (o_a o_b - o_a p_a o_b p_b) (...). */
+template <int TP, int G, bool MIXED, bool PRED, bool CORR>
+__device__ __forceinline__ uint64_t vf_tile(const VfTileLds<TP>& px, const VfLane<G>& L, const int (&bt)[G],
+                                            const double2* tab,
+                                            typename std::conditional<MIXED, vw_f4, vw_d4>::type (&y11)[VfShape<G>::NS],
+                                            typename std::conditional<MIXED, vw_f4, vw_d4>::type (&y22)[VfShape<G>::NS],
+                                            typename std::conditional<MIXED, vw_f4, vw_d4>::type (&y21)[G * G]) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  constexpr bool NEED_N = PRED || CORR;
+  const int q = (threadIdx.x & 63) >> 4;
+  bool vis[G], allv[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    vis[gi] = (bt[gi] & 2) == 0;
+    allv[gi] = (bt[gi] & 1) != 0;
+  }
+  uint64_t failed = 0;
+  for (int st = 0; st < TP / 4; ++st) {
+    const int e = 4 * st + q;
+    const double l = px.l[e], m = px.m[e], nm1 = px.nm1[e];
+    const double n = NEED_N ? px.n[e] : 0.0;
+    const T w = (T)px.w[e];
+    T cm[G], sm[G], co[G], so[G];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      bool msk, omk = false;
+      if (PRED || CORR) {
+        const bool pv = vw_visible(l, m, n, L.px[gi], L.py[gi], L.pz[gi], L.pinv[gi], L.pcos[gi]);
+        if (CORR) {
+          omk = allv[gi];
+          msk = allv[gi] & pv;
+        } else {
+          msk = vis[gi] & pv;
+          failed |= ~__ballot(pv);
+        }
+      } else {
+        msk = vis[gi];
+      }
+      T c, s;
+      if (MIXED) {
+        /* coefficients in turns for this form; the fraction in [0, 1) */
+        const double x = vw_viss_phase(L.kx[gi], L.ky[gi], L.kz[gi], l, m, nm1);
+        float sf, cf;
+        vw_hw_sincos_turn((float)__builtin_amdgcn_fract(x), &sf, &cf);
+        c = cf;
+        s = sf;
+      } else {
+        int ix;
+        const double f = vw_tab_split(vw_viss_phase(L.kx[gi], L.ky[gi], L.kz[gi], l, m, nm1), &ix);
+        const double2 te = tab[ix];
+        double sy, cy, sd, cd;
+        vw_tab_poly(f, &sy, &cy);
+        vw_tab_rotate(te.x, te.y, sy, cy, &sd, &cd);
+        c = (T)cd;
+        s = (T)sd;
+      }
+      cm[gi] = msk ? c : (T)0;
+      sm[gi] = msk ? s : (T)0;
+      co[gi] = omk ? c : (T)0;
+      so[gi] = omk ? s : (T)0;
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      /* CORR: the second product of each block carries the minus sign */
+      const T wm = CORR ? -w : w;
+      const T wc = wm * cm[gi], ws = wm * sm[gi];
+      const T wco = w * co[gi], wso = w * so[gi];
+#pragma unroll
+      for (int gj = 0; gj <= gi; ++gj) {
+        const int b = gi * (gi + 1) / 2 + gj;
+        if (CORR) {
+          y11[b] = vw_mfma(wc, cm[gj], vw_mfma(wco, co[gj], y11[b]));
+          y22[b] = vw_mfma(ws, sm[gj], vw_mfma(wso, so[gj], y22[b]));
+        } else {
+          y11[b] = vw_mfma(wc, cm[gj], y11[b]);
+          y22[b] = vw_mfma(ws, sm[gj], y22[b]);
+        }
+      }
+#pragma unroll
+      for (int gj = 0; gj < G; ++gj) {
+        if (CORR) y21[gi * G + gj] = vw_mfma(ws, cm[gj], vw_mfma(wso, co[gj], y21[gi * G + gj]));
+        else y21[gi * G + gj] = vw_mfma(ws, cm[gj], y21[gi * G + gj]);
+      }
+    }
+  }
+  return failed;
+}
+
+#ifndef VW_VISSF_WAVES_G2
+#define VW_VISSF_WAVES_G2 2
+#endif
+template <int TP, int G, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 : VW_VISSF_WAVES_G2))) void k_vissf(
+    VissFArgs a) {
+  using Acc = typename std::conditional<MIXED, vw_f4, vw_d4>::type;
+  constexpr int NS = VfShape<G>::NS, NB = VfShape<G>::NB;
+  constexpr int EPT = TP / 256; /* pixels staged per thread */
+  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
+  __shared__ VfTileLds<TP> px;
+  if (!MIXED) vw_tab_fill(tab);
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t chunk = vb / a.nquad;
+  if (chunk >= a.nchunk) return; /* whole workgroup */
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int64_t g = (vb - chunk * a.nquad) * 4 + wave; /* epoch of this wave */
+  /* waves past the last epoch stage pixels and meet the barriers, nothing else */
+  const bool active = g < a.tlen;
+  const int64_t t0 = a.tile0 + chunk * a.tpc;
+  const int64_t tend = a.tile0 + a.ntile;
+  const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
+
+  VfLane<G> L;
+  int64_t fr[G];
+  bool have[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const int ant = gi * 16 + (lane & 15);
+    have[gi] = active && ant < a.A;
+    fr[gi] = have[gi] ? g * a.A + ant : 0;
+    const double4 k = a.aph[fr[gi]];
+    const double* q8 = a.ant8 + fr[gi] * 8;
+    /* 1/1024 turns for the table sincos, turns (exact rescaling) for the hardware one */
+    const double sc = MIXED ? 1.0 / VW_TAB_SCALE : 1.0;
+    L.kx[gi] = have[gi] ? k.x * sc : 0.0;
+    L.ky[gi] = have[gi] ? k.y * sc : 0.0;
+    L.kz[gi] = have[gi] ? k.z * sc : 0.0;
+    L.px[gi] = q8[0]; L.py[gi] = q8[1]; L.pz[gi] = q8[2]; L.pinv[gi] = q8[3]; L.pcos[gi] = q8[7];
+  }
+  /* the plain form sums in the FP64 accumulators of the matrix cores; the
+   * mixed form sums each tile in FP32 and adds it to FP64 totals */
+  Acc y11[NS], y22[NS], y21[G * G];
+  vw_d4 s11[NS], s22[NS], s21[G * G];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) { y11[i] = Acc{0, 0, 0, 0}; y22[i] = Acc{0, 0, 0, 0}; s11[i] = vw_d4{0, 0, 0, 0}; s22[i] = vw_d4{0, 0, 0, 0}; }
+#pragma unroll
+  for (int i = 0; i < G * G; ++i) { y21[i] = Acc{0, 0, 0, 0}; s21[i] = vw_d4{0, 0, 0, 0}; }
+
+  /* pixel data of the next tile, loaded one tile ahead */
+  double nl[EPT], nm[EPT], nn[EPT], nz[EPT], nw[EPT];
+  auto fetch = [&](int64_t tile) {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int64_t p = tile * TP + threadIdx.x + 256 * k;
+      const bool ok = p < a.npix;
+      /* past the end of the sky: the tile's first pixel with weight zero, so
+       * the predicate test sees only the tile's own directions */
+      const int64_t pp = ok ? p : tile * TP;
+      nl[k] = a.l[pp]; nm[k] = a.m[pp]; nn[k] = a.n[pp]; nz[k] = a.nm1[pp];
+      nw[k] = ok ? (MIXED ? (double)a.Bf[pp] : a.B[pp]) : 0.0;
+    }
+  };
+  fetch(t0);
+  for (int64_t tile = t0; tile < t1; ++tile) {
+    __syncthreads(); /* the previous tile's readers are done (and the table is built) */
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      px.l[e] = nl[k]; px.m[e] = nm[k]; px.n[e] = nn[k]; px.nm1[e] = nz[k]; px.w[e] = nw[k];
+    }
+    __syncthreads();
+    if (tile + 1 < t1) fetch(tile + 1);
+    if (!active) continue;
+    const uint8_t* brow = a.bits + (tile - a.tile0) * a.nfr_pad;
+    int bt[G];
+    int nvis = 0;
+    bool any_mixed = false;
+    uint64_t allvis = 0; /* lanes of all-visible antennas, any group */
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      bt[gi] = have[gi] ? (int)brow[fr[gi]] : 2;
+      nvis += __popcll(__ballot((bt[gi] & 2) == 0) & 0xffffull);
+      any_mixed |= (__ballot((bt[gi] & 3) == 0) & 0xffffull) != 0;
+      allvis |= __ballot((bt[gi] & 1) != 0);
+    }
+    /* fewer than two antennas see any of the tile: every pair is hidden */
+    if (nvis < 2) continue;
+    if (!any_mixed) {
+      vf_tile<TP, G, MIXED, false, false>(px, L, bt, tab, y11, y22, y21);
+    } else {
+      const uint64_t failed = vf_tile<TP, G, MIXED, true, false>(px, L, bt, tab, y11, y22, y21);
+      if (a.check || (failed & allvis) != 0) vf_tile<TP, G, MIXED, true, true>(px, L, bt, tab, y11, y22, y21);
+    }
+    if (MIXED) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          s11[i][r] += (double)y11[i][r];
+          s22[i][r] += (double)y22[i][r];
+          y11[i][r] = 0;
+          y22[i][r] = 0;
+        }
+#pragma unroll
+      for (int i = 0; i < G * G; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          s21[i][r] += (double)y21[i][r];
+          y21[i][r] = 0;
+        }
+    }
+  }
+  if (!active) return;
+  /* partial blocks: Re = Y11 + Y22 for gi >= gj, then Y21; row-major 16 x 16 */
+  double* out = a.part + (g * a.nchunk + chunk) * (int64_t)NB * 256;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = MIXED ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r, j = lane & 15;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      out[s * 256 + i * 16 + j] = MIXED ? s11[s][r] + s22[s][r] : (double)y11[s][r] + (double)y22[s][r];
+#pragma unroll
+    for (int s = 0; s < G * G; ++s) out[(NS + s) * 256 + i * 16 + j] = MIXED ? s21[s][r] : (double)y21[s][r];
+  }
+}
+
+/* partial blocks -> raw half sums of the epoch's pairs (pa[j], pb[j]), chunks
+ * folded in ascending order; passes after the first add to the running sums */
+__global__ void k_vissf_reduce(const double* __restrict__ part, int64_t nchunk, int64_t nh, int P, int G,
+                               const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int accumulate,
+                               double2* __restrict__ vraw) {
+  int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const int64_t g = h / P;
+  const int j = (int)(h - g * P);
+  const int ea = pa[j], eb = pb[j];
+  const int hi = ea > eb ? ea : eb, lo = ea > eb ? eb : ea;
+  const int NS = G * (G + 1) / 2, NB = NS + G * G;
+  const int64_t ire = (int64_t)((hi >> 4) * ((hi >> 4) + 1) / 2 + (lo >> 4)) * 256 + (hi & 15) * 16 + (lo & 15);
+  const int64_t iab = (int64_t)(NS + (ea >> 4) * G + (eb >> 4)) * 256 + (ea & 15) * 16 + (eb & 15);
+  const int64_t iba = (int64_t)(NS + (eb >> 4) * G + (ea >> 4)) * 256 + (eb & 15) * 16 + (ea & 15);
+  double re = 0.0, im = 0.0;
+  for (int64_t k = 0; k < nchunk; ++k) {
+    const double* b = part + (g * nchunk + k) * (int64_t)NB * 256;
+    re += b[ire];
+    im += b[iab] - b[iba];
+  }
+  if (accumulate) {
+    double2 o = vraw[h];
+    re = o.x + re;
+    im = o.y + im;
+  }
+  vraw[h] = make_double2(re, im);
+}
+
+/* per-frame phase coefficients of the factorized form: (x - x_0) * 1e3 /
+ * lambda * 1024 with x_0 the epoch's antenna 0 */
+__global__ void k_ant_phase(const double* __restrict__ pos, int T, int A, int t0, int tlen, double lambda,
+                            double4* __restrict__ aph) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)tlen * A) return;
+  const int64_t TA = (int64_t)T * A;
+  const int64_t g = k / A;
+  const int64_t src = (int64_t)t0 * A + k, ref = (t0 + g) * A;
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = (((pos[c * TA + src] - pos[c * TA + ref]) * 1e3) / lambda) * VW_TAB_SCALE;
+  aph[k] = make_double4(r[0], r[1], r[2], 0.0);
 }
 
 /* phase correction + conjugate completion (common.hpp:255-277) */
