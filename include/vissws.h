@@ -43,7 +43,7 @@ extern "C" {
 
 /* Viss summation form (vw_config.viss_form) */
 #define VW_VISS_AUTO 0      /* per-row sums in the plain form; antenna-factorized in the mixed form
-                             * for n_ant <= 16                                                   */
+                             * for n_ant <= 32                                                   */
 #define VW_VISS_ROWS 1      /* one phase / sin / cos term per (half row, pixel): the reference's
                              * phase rounding, bit for bit                                       */
 #define VW_VISS_ANT 2       /* antenna-factorized on the matrix cores in either form (n_ant <= 32);

@@ -208,10 +208,10 @@ ANT_PLAIN_TOL = 1e-7
 )
 def test_viss_forms(n_ant, form, precision):
     # the per-row form (viss_form 1), the antenna-factorized form (2; the mixed
-    # form's default, 0, up to 16 antennas) with one (n_ant <= 16) or two
+    # form's default, 0, up to 32 antennas) with one (n_ant <= 16) or two
     # (17..32) groups of 16 antennas, its correction pass on every tile with a
     # partly visible antenna (3), and the per-row form the mixed default takes
-    # above 16 antennas
+    # above 32 antennas
     cfg = config(nside=32, n_ant=n_ant, n_epochs=12, segs=2, img_n=0, viss_form=form, precision=precision,
                  freq_hz=2e7)
     tol = MIXED_TOL if precision else (PLAIN_TOL if form == 1 else ANT_PLAIN_TOL)

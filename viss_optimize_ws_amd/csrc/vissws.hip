@@ -218,12 +218,11 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   c->ntile = (c->npix + c->TP - 1) / c->TP;
   c->A = cfg->n_ant;
   c->P = c->A * (c->A - 1) / 2;
-  /* the automatic choice takes the factorized form where it measured faster:
-   * the mixed form with one group of 16 antennas (Nside 128, 16 antennas:
-   * 0.16 against 0.30 ms per segment); with two groups the ten products per
-   * step cost as much as the per-row terms */
+  /* automatic choice: the factorized form for the mixed precision (per
+   * segment, Nside 128 / 16 antennas 0.16 against 0.30 ms, Nside 256 / 32
+   * antennas 3.0 against 7.0 ms); the plain form keeps the per-row phases */
   c->vfact = c->A <= 32 && (cfg->viss_form == VW_VISS_ANT || cfg->viss_form == VW_VISS_ANT_CHECK ||
-                            (cfg->viss_form == VW_VISS_AUTO && cfg->precision == VW_PREC_MIXED && c->A <= 16));
+                            (cfg->viss_form == VW_VISS_AUTO && cfg->precision == VW_PREC_MIXED));
   c->lambda = 3e8 / cfg->freq_hz;
   c->pix_area = (4.0 * VW_PI) / (double)c->npix;
   c->per_ant = cfg->occ_altitude_km < 0.0;

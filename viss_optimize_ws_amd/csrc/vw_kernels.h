@@ -603,8 +603,8 @@ template <int G> struct VfLane {
 
 /* the product-mask sum of one tile.  PRED false (no partly visible
  * antenna in the tile): m = hidden ? 0 : 1.  PRED: m = hidden ? 0 :
- * predicate, and the return value holds the lanes whose predicate failed on
- * some pixel (a lane mask; only all-visible antennas' lanes matter).  CORR:
+ * predicate, and the return value is nonzero when the predicate of an
+ * all-visible antenna failed on some pixel of the tile.  CORR:
  * the correction pass instead, for pairs of two all-visible antennas
  * (o = all-visible): Y += sum w (o_a o_b - o_a p_a o_b p_b) (...). */
 template <int TP, int G, bool MIXED, bool PRED, bool CORR>
@@ -617,10 +617,12 @@ __device__ __forceinline__ uint64_t vf_tile(const VfTileLds<TP>& px, const VfLan
   constexpr bool NEED_N = PRED || CORR;
   const int q = (threadIdx.x & 63) >> 4;
   bool vis[G], allv[G];
+  uint64_t allm[G]; /* lanes whose antenna of group gi is all-visible */
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) {
     vis[gi] = (bt[gi] & 2) == 0;
     allv[gi] = (bt[gi] & 1) != 0;
+    allm[gi] = __ballot(allv[gi]);
   }
   uint64_t failed = 0;
   for (int st = 0; st < TP / 4; ++st) {
@@ -639,7 +641,7 @@ __device__ __forceinline__ uint64_t vf_tile(const VfTileLds<TP>& px, const VfLan
           msk = allv[gi] & pv;
         } else {
           msk = vis[gi] & pv;
-          failed |= ~__ballot(pv);
+          failed |= ~__ballot(pv) & allm[gi];
         }
       } else {
         msk = vis[gi];
@@ -772,13 +774,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 
     int bt[G];
     int nvis = 0;
     bool any_mixed = false;
-    uint64_t allvis = 0; /* lanes of all-visible antennas, any group */
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       bt[gi] = have[gi] ? (int)brow[fr[gi]] : 2;
       nvis += __popcll(__ballot((bt[gi] & 2) == 0) & 0xffffull);
       any_mixed |= (__ballot((bt[gi] & 3) == 0) & 0xffffull) != 0;
-      allvis |= __ballot((bt[gi] & 1) != 0);
     }
     /* fewer than two antennas see any of the tile: every pair is hidden */
     if (nvis < 2) continue;
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 
       vf_tile<TP, G, MIXED, false, false>(px, L, bt, tab, y11, y22, y21);
     } else {
       const uint64_t failed = vf_tile<TP, G, MIXED, true, false>(px, L, bt, tab, y11, y22, y21);
-      if (a.check || (failed & allvis) != 0) vf_tile<TP, G, MIXED, true, true>(px, L, bt, tab, y11, y22, y21);
+      if (a.check || failed != 0) vf_tile<TP, G, MIXED, true, true>(px, L, bt, tab, y11, y22, y21);
     }
     if (MIXED) {
 #pragma unroll
