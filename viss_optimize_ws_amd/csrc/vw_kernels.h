@@ -349,7 +349,7 @@ __device__ __forceinline__ bool vw_vis2(bool ok, double l, double m, double n, d
 __device__ __forceinline__ void vw_viss_term_mixed(const RowRec& rec, double l, double m, double nm1, float bf,
                                                    float& fre, float& fim) {
   const double x = vw_viss_phase(rec.u, rec.v, rec.w, l, m, nm1);
-  const float f = (float)(x - rint(x));
+  const float f = (float)__builtin_amdgcn_fract(x);
   float s, c;
   vw_hw_sincos_turn(f, &s, &c);
   fre = fmaf(bf, c, fre);
@@ -1136,8 +1136,9 @@ __device__ __forceinline__ void vw_ant_vis(const double* __restrict__ ant8, int 
 struct ReconAccF32 {
   float s = 0.f, c = 0.f, b = 0.f;
   __device__ void term(double ugu, double vgu, float vrf, float vif, double lp, double mp) {
+    /* the fraction in [0, 1) turns: one v_fract_f64 */
     const double x = vw_recon_phase(ugu, vgu, lp, mp);
-    const float f = (float)(x - rint(x));
+    const float f = (float)__builtin_amdgcn_fract(x);
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
     b = fmaf(vrf, cs, fmaf(-vif, sn, b));
   }
