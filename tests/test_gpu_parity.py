@@ -231,3 +231,26 @@ def test_viss_forms(n_ant, form, precision):
             ho, _ = o.conj_fill()
             assert rel_err(hg, ho) <= tol
     print(f"viss_form {form} n_ant {n_ant} precision {precision}: max rel err {max(errs):.3e}")
+
+
+@pytest.mark.parametrize("variant", [dict(tile_pix=512), dict(class_budget_bytes=1), dict(seg_tlen=[5, 13, 3])],
+                         ids=["tp512", "one_tile_per_pass", "uneven_segments"])
+def test_viss_factorized_shapes(variant):
+    # the factorized form (FP64 and mixed) on 512-pixel tiles, with one tile
+    # per pass (partials folded across passes), and on uneven segments whose
+    # epoch counts leave workgroups with idle waves
+    kw = dict(nside=32, n_ant=12, img_n=0, freq_hz=2e7)
+    if "seg_tlen" not in variant:
+        kw.update(n_epochs=12, segs=2)
+    kw.update(variant)
+    for precision, form, tol in ((0, 2, ANT_PLAIN_TOL), (1, 0, MIXED_TOL)):
+        cfg = config(viss_form=form, precision=precision, **kw)
+        sky = _sky(12 * 32 * 32, seed=21)
+        with Pipeline(cfg) as g, Oracle(cfg) as o:
+            g.set_sky(sky)
+            o.set_sky(sky)
+            for t0, tl in g.schedule():
+                g.gen_segment(t0, tl)
+                o.gen_segment(t0, tl)
+                err = rel_err(g.viss_half(), o.viss_half())
+                assert err <= tol, (variant, precision, t0, err)

@@ -98,6 +98,7 @@ struct Ctx {
   bool vfact = false;
   double4* d_aph = nullptr;
   double* d_fpart = nullptr;
+  double* d_fesum = nullptr; /* [tlen][block][256] epoch sums of the partial blocks */
   int64_t fpart_cap = 0;
   int64_t* d_counts = nullptr;
   double2 *d_vraw = nullptr, *d_vh = nullptr, *d_vfull = nullptr;
@@ -547,7 +548,8 @@ int create_device(Ctx* c) {
         fp = std::max(fp, (int64_t)c->seg_tlen[k] * viss_fshape(c, c->seg_tlen[k], nt).nchunk *
                               vissf_blocks(vissf_groups(c->A)) * 256);
     c->fpart_cap = fp;
-    if ((rc = dalloc(c, &c->d_fpart, (size_t)fp)) || (rc = dalloc(c, &c->d_aph, (size_t)c->max_tlen * c->A)))
+    if ((rc = dalloc(c, &c->d_fpart, (size_t)fp)) || (rc = dalloc(c, &c->d_aph, (size_t)c->max_tlen * c->A)) ||
+        (rc = dalloc(c, &c->d_fesum, (size_t)c->max_tlen * vissf_blocks(vissf_groups(c->A)) * 256)))
       return rc;
   }
   for (int b = 0; b < 2; ++b) {
@@ -705,8 +707,11 @@ int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
 #undef VW_VF
   KC();
   ev_end(c, ST_VISSK);
-  k_vissf_reduce<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_fpart, s.nchunk, c->nh, c->P, G, c->d_pa, c->d_pb,
-                                                       tile0 > 0 ? 1 : 0, c->d_vraw);
+  const int64_t nent = vissf_blocks(G) * 256;
+  k_vissf_fold<<<(unsigned)(c->tlen * (nent / 32)), 256, 0, c->st>>>(c->d_fpart, s.nchunk, nent, c->d_fesum);
+  KC();
+  k_vissf_rows<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_fesum, c->nh, c->P, G, c->d_pa, c->d_pb, tile0 > 0 ? 1 : 0,
+                                                     c->d_vraw);
   KC();
   return VW_OK;
 }

@@ -821,11 +821,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 
   }
 }
 
-/* partial blocks -> raw half sums of the epoch's pairs (pa[j], pb[j]), chunks
- * folded in ascending order; passes after the first add to the running sums */
-__global__ void k_vissf_reduce(const double* __restrict__ part, int64_t nchunk, int64_t nh, int P, int G,
-                               const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int accumulate,
-                               double2* __restrict__ vraw) {
+/* partial blocks -> per-epoch block sums.  Workgroup (epoch, slice of 32
+ * entries); its eight 32-lane groups take every eighth chunk (256-byte
+ * reads), then the eight sums are added in a fixed order (deterministic). */
+__global__ __launch_bounds__(256) void k_vissf_fold(const double* __restrict__ part, int64_t nchunk, int64_t nent,
+                                                    double* __restrict__ esum) {
+  __shared__ double red[8][32];
+  const int64_t nslice = nent / 32;
+  const int64_t g = blockIdx.x / nslice, sl = blockIdx.x - g * nslice;
+  const int t = threadIdx.x & 31, c0 = threadIdx.x >> 5;
+  const int64_t e = sl * 32 + t;
+  double s = 0.0;
+  for (int64_t k = c0; k < nchunk; k += 8) s += part[(g * nchunk + k) * nent + e];
+  red[c0][t] = s;
+  __syncthreads();
+  if (c0 == 0) {
+    double r = red[0][t];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) r += red[i][t];
+    esum[g * nent + e] = r;
+  }
+}
+
+/* per-epoch block sums -> raw half sums of the epoch's pairs (pa[j], pb[j]);
+ * passes after the first add to the running sums */
+__global__ void k_vissf_rows(const double* __restrict__ esum, int64_t nh, int P, int G,
+                             const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int accumulate,
+                             double2* __restrict__ vraw) {
   int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   const int64_t g = h / P;
@@ -836,12 +858,8 @@ __global__ void k_vissf_reduce(const double* __restrict__ part, int64_t nchunk, 
   const int64_t ire = (int64_t)((hi >> 4) * ((hi >> 4) + 1) / 2 + (lo >> 4)) * 256 + (hi & 15) * 16 + (lo & 15);
   const int64_t iab = (int64_t)(NS + (ea >> 4) * G + (eb >> 4)) * 256 + (ea & 15) * 16 + (eb & 15);
   const int64_t iba = (int64_t)(NS + (eb >> 4) * G + (ea >> 4)) * 256 + (eb & 15) * 16 + (ea & 15);
-  double re = 0.0, im = 0.0;
-  for (int64_t k = 0; k < nchunk; ++k) {
-    const double* b = part + (g * nchunk + k) * (int64_t)NB * 256;
-    re += b[ire];
-    im += b[iab] - b[iba];
-  }
+  const double* b = esum + g * (int64_t)NB * 256;
+  double re = b[ire], im = b[iab] - b[iba];
   if (accumulate) {
     double2 o = vraw[h];
     re = o.x + re;
