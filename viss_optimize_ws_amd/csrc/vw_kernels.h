@@ -625,6 +625,9 @@ __device__ __forceinline__ uint64_t vf_tile(const VfTileLds<TP>& px, const VfLan
     allm[gi] = __ballot(allv[gi]);
   }
   uint64_t failed = 0;
+  /* not unrolled: the mixed form then fits 5 waves per SIMD (measured 4 %
+   * faster than the compiler's two-step unroll at 4 waves) */
+#pragma unroll 1
   for (int st = 0; st < TP / 4; ++st) {
     const int e = 4 * st + q;
     const double l = px.l[e], m = px.m[e], nm1 = px.nm1[e];
