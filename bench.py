@@ -11,9 +11,12 @@ the device.  Inputs (sky, resident directions) are in HBM before timing.
 
 Metric: pixel-baseline Gterms/s = (sum_seg npix*segN_half +
 sum_seg (npix + n_img)*nkeys_seg) / t / 1e9, counting culled terms too.
-The mixed form (FP64 phase, FP32 hardware sincos, FP32 block sums folded in
-FP64) is timed on the same workload and its error is stated against the
-plain FP64 result (which matches the CPU FP64 oracle to 1e-9).
+The mixed form (antenna-factorized Viss on the FP32 matrix cores with FP64
+antenna phases; recon with FP64 phase, FP32 hardware sincos, FP32 block sums
+folded in FP64) is timed on the same workload and its error is stated
+against the plain FP64 result (which matches the CPU FP64 oracle to 1e-9).
+So is the FP64 antenna-factorized Viss option (viss_form 2, "plain_ant"),
+whose phases differ from the per-row ones by FP64 rounding.
 """
 import argparse
 import json
@@ -73,10 +76,10 @@ def _traffic(workload, kernel):
 SERIAL = False
 
 
-def _cfg(precision):
+def _cfg(precision, viss_form=0):
     from viss_optimize_ws_amd import default_config
 
-    return default_config(1, precision=precision, serial_stages=int(SERIAL), **WORKLOAD)
+    return default_config(1, precision=precision, serial_stages=int(SERIAL), viss_form=viss_form, **WORKLOAD)
 
 
 def _cfg_serial(precision):
@@ -203,6 +206,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-ant", action="store_true", help="skip the FP64 antenna-factorized Viss leg")
     ap.add_argument("--serial", action="store_true",
                     help="run every stage on one stream (no overlap of a segment's recon with the next Viss)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nside128",
@@ -318,9 +322,22 @@ def main():
             from_err[name] = {"max_rel": float(np.max(np.abs(a - b)) / den),
                               "rms_rel": float(np.sqrt(np.mean(np.abs(a - b) ** 2)) / np.sqrt(np.mean(np.abs(b) ** 2)))}
         out["mixed"] = {"value": round(terms / sec_m / 1e9, 4), "unit": "Gterms/s", "ms_per_step": round(sec_m * 1e3, 4),
-                        "speedup_vs_plain": round(sec / sec_m, 3), "dtype": "f64 phase + f32 sincos/sum",
+                        "speedup_vs_plain": round(sec / sec_m, 3),
+                        "dtype": "f64 phase + f32 sincos/sum (Viss: f32 matrix-core products of antenna phasors)",
                         "err_vs_fp64": from_err,
                         "viss_kernel_avg_ms": round(tim_m_iso["viss_kernel_ms"] / max(tim_m_iso["n_viss_launch"], 1), 4)}
+    if not args.no_ant:
+        # FP64 throughout, Viss by the antenna-factorized form on the FP64 matrix cores
+        with Pipeline(_cfg(0, viss_form=2)) as ant:
+            ant.set_sky(sky)
+            per_seg_a, _, tim_a_iso = _plan_counts(ant)
+            sec_a, _ = _time_steps(ant, args.steps, args.warmup)
+        vr_a = np.concatenate([s["vraw"] for s in per_seg_a])
+        vr_p = np.concatenate([s["vraw"] for s in per_seg])
+        out["plain_ant"] = {"value": round(terms / sec_a / 1e9, 4), "unit": "Gterms/s",
+                            "ms_per_step": round(sec_a * 1e3, 4), "dtype": "f64",
+                            "viss_kernel_avg_ms": round(tim_a_iso["viss_kernel_ms"] / max(tim_a_iso["n_viss_launch"], 1), 4),
+                            "viss_max_rel_err_vs_per_row": float(np.max(np.abs(vr_a - vr_p)) / np.max(np.abs(vr_p)))}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = _cpu_baseline(sky, threads, per_seg[0]["vraw"], args.workload == "nside128")
