@@ -254,3 +254,20 @@ def test_viss_factorized_shapes(variant):
                 o.gen_segment(t0, tl)
                 err = rel_err(g.viss_half(), o.viss_half())
                 assert err <= tol, (variant, precision, t0, err)
+
+
+def test_mixed_lunar_factorized():
+    # the reference lunar formation (8 antennas, positions ~2,000 km from the
+    # Moon's centre, global occultation cone) in the mixed form, whose Viss is
+    # the antenna-factorized kernel with 8 of 16 antenna rows in use
+    cfg = config(geometry=0, nside=8, freq_hz=1e7, segs=20, img_n=32, precision=1)
+    assert cfg.viss_form == 0
+    errs = []
+
+    def chk(g, o, k, d):
+        errs.append(rel_err(d["vg"], d["vo"]))
+
+    (Cg, Wg, Ig, WIg), (Co, Wo, Io, WIo) = run_both(cfg, _sky(12 * 8 * 8, seed=6), chk, segments={0, 9, 19})
+    assert max(errs) <= MIXED_TOL, errs
+    assert np.array_equal(Wg, Wo)
+    assert rel_err(Cg, Co) <= MIXED_TOL and rel_err(Ig, Io) <= MIXED_TOL

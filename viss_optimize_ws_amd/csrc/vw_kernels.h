@@ -11,6 +11,10 @@
  *    writes a partial that a second kernel folds in chunk order
  *    (deterministic).  The plain form evaluates sin/cos from a 1024-entry
  *    LDS table (vw_shared.h).
+ *  - Viss, antenna-factorized form (the mixed form's default): a wave owns
+ *    one epoch and a chunk of tiles, lanes hold (antenna, pixel) pairs, and
+ *    each step of 4 pixels feeds per-antenna exponentials to 16x16x4 matrix
+ *    core products whose accumulators are the epoch's pair sums.
  *  - Recon: lanes hold output points (4 per lane, one output tile per wave),
  *    groups are wave-uniform and pass through a per-wave LDS ring; the group
  *    range of a segment is split into chunks whose partials a second kernel
