@@ -89,6 +89,15 @@ typedef struct {
                                 * for profiling single kernels)                     */
   int32_t viss_form;           /* VW_VISS_*: summation form of the Viss stage (the
                                 * half visibilities agree to FP64 rounding)         */
+  int32_t nsky;                /* sky realisations that share the geometry and the
+                                * segment schedule (0 or 1: one sky).  Above 1 the
+                                * context runs the batched form: mixed precision
+                                * only; Viss and recon as BF16 hi/lo matrix-core
+                                * products of on-the-fly phasor tiles against the
+                                * sky / group-visibility matrix (FP32 accumulate,
+                                * FP64 across blocks); recon rules with one member
+                                * per group (occ_mode 1, 2) or blockage 0.  No
+                                * counterpart in the reference, which runs one sky */
 } vw_config;
 
 /* Derived plan (filled by vw_get_plan). */
@@ -165,7 +174,15 @@ int vw_set_day(void* ctx, int32_t day);
 /* ---------------- stages ---------------- */
 
 /* Sky intensities B[nsky][npix] (NESTED).  Scaled by pix_area = 4 pi / npix
- * (main_recon_online.cu:105-110).  nsky must be 1. */
+ * (main_recon_online.cu:105-110).  nsky must equal the configured nsky (1
+ * unless batched); batched skies must be finite.
+ *
+ * Batched contexts (nsky > 1) keep every stage call and its order; the
+ * per-sky outputs are stacked: vw_viss_half and vw_conj_fill write
+ * [nsky][n_half] (full rows [nsky][2 n_half]); vw_finalize writes
+ * C_healpix [nsky][npix] and img [nsky][img_n^2] with one weight map
+ * W_healpix [npix] / W_img [img_n^2] (the weights do not depend on the sky);
+ * vw_get/set_accumulators take C [nsky][npix] and G [nsky][n_img_pts]. */
 int vw_set_sky(void* ctx, const double* B, int64_t npix, int32_t nsky);
 
 /* Generates epochs [t0, t0+tlen): antenna positions, per-antenna occlusion

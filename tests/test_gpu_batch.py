@@ -1,0 +1,142 @@
+"""Batched skies (nsky > 1) through the C-ABI, against the CPU FP64 oracle
+and against the one-sky plain FP64 path on the GPU (needs a GPU).
+
+The batch shares geometry, schedule and groups, so the weight maps (which
+depend only on the geometry) must equal the one-sky weights bit for bit;
+visibilities and maps of every sky are held to the mixed-form bar
+max|x - ref| / max|ref| <= 1e-4 (MIXED_TOL).
+"""
+import numpy as np
+import pytest
+
+from _lockstep import MIXED_TOL, config, rel_err
+from oracle.oracle import Oracle
+from viss_optimize_ws_amd import Pipeline, VissError, make_sky
+
+pytestmark = pytest.mark.gpu
+
+
+def _skies(nsky, npix, seed):
+    """alternating log-normal and log-normal + point-source skies"""
+    return np.stack([make_sky(seed + s, npix, s % 2) for s in range(nsky)])
+
+
+def test_batch_lockstep_oracle():
+    """every stage of every segment, per sky, against one oracle per sky"""
+    nsky = 20  # two 16-sky blocks per wave, 12 padding skies
+    kw = dict(nside=16, n_ant=8, n_epochs=16, segs=2, img_n=48)
+    cfg = config(precision=1, nsky=nsky, **kw)
+    npix = 12 * 16 * 16
+    skies = _skies(nsky, npix, 100)
+    oracles = [Oracle(config(**kw)) for _ in range(nsky)]
+    try:
+        with Pipeline(cfg) as g:
+            g.set_sky(skies)
+            for s, o in enumerate(oracles):
+                o.set_sky(skies[s])
+            for t0, tl in g.schedule():
+                g.gen_segment(t0, tl)
+                vg = g.viss_half()
+                hg, fg = g.conj_fill()
+                ig = g.recon_segment()
+                assert vg.shape == (nsky, g.nh) and fg.shape == (nsky, 2 * g.nh)
+                for s, o in enumerate(oracles):
+                    o.gen_segment(t0, tl)
+                    vo = o.viss_half()
+                    ho, fo = o.conj_fill()
+                    io = o.recon_segment()
+                    assert rel_err(vg[s], vo) <= MIXED_TOL, (s, rel_err(vg[s], vo))
+                    assert rel_err(hg[s], ho) <= MIXED_TOL
+                    assert rel_err(fg[s], fo) <= MIXED_TOL
+                    assert ig["nuniq"] == io["nuniq"] and ig["nkeys"] == io["nkeys"]
+            Cg, Wg, Ig, WIg = g.finalize()
+        assert Cg.shape == (nsky, npix) and Ig.shape == (nsky, 48, 48)
+        for s, o in enumerate(oracles):
+            Co, Wo, Io, WIo = o.finalize()
+            assert np.array_equal(Wg, Wo) and np.array_equal(WIg, WIo)
+            assert rel_err(Cg[s], Co) <= MIXED_TOL, (s, rel_err(Cg[s], Co))
+            assert rel_err(Ig[s], Io) <= MIXED_TOL, (s, rel_err(Ig[s], Io))
+    finally:
+        for o in oracles:
+            o.close()
+
+
+CASES = {
+    # 64 skies: four 16-sky blocks per wave
+    "nsky64": dict(nside=32, n_ant=16, n_epochs=32, segs=2, img_n=64, nsky=64),
+    # three blocks per wave, full uv grid, first-member rule
+    "full_occ1": dict(nside=32, n_ant=8, n_epochs=24, segs=3, img_n=40, nsky=33, recon_mode=1, occ_mode=1),
+    # blockage 0: group means, every group counts, W_total normalisation
+    "blockage0": dict(nside=16, n_ant=8, n_epochs=16, segs=2, img_n=32, nsky=17, blockage=0),
+    # 512-pixel tiles and uneven explicit segments
+    "tp512_uneven": dict(nside=32, n_ant=12, n_epochs=40, img_n=48, nsky=40, tile_pix=512, seg_tlen=[5, 19, 16]),
+    # a small class budget: several tile passes per stage, one block per wave
+    "tile_passes": dict(nside=32, n_ant=8, n_epochs=16, segs=2, img_n=48, nsky=16, class_budget_bytes=64 * 1024),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_batch_vs_one_sky_plain(case):
+    kw = dict(CASES[case])
+    nsky = kw.pop("nsky")
+    npix = 12 * kw["nside"] ** 2
+    skies = _skies(nsky, npix, 7)
+    with Pipeline(config(precision=1, nsky=nsky, **kw)) as g:
+        g.set_sky(skies)
+        t0, tl = g.schedule()[0]
+        g.gen_segment(t0, tl)
+        v0 = g.viss_half()
+        g.reset_accumulators()
+        g.run_all()
+        Cg, Wg, Ig, WIg = g.finalize()
+    # the one-sky plain FP64 path on a subset of the skies (first, last, two inside)
+    for s in sorted({0, 1, nsky // 2, nsky - 1}):
+        with Pipeline(config(**kw)) as p:
+            p.set_sky(skies[s])
+            p.gen_segment(t0, tl)
+            vp = p.viss_half()
+            assert rel_err(v0[s], vp) <= MIXED_TOL, (case, s, rel_err(v0[s], vp))
+            p.reset_accumulators()
+            p.run_all()
+            Cp, Wp, Ip, WIp = p.finalize()
+        assert np.array_equal(Wg, Wp) and np.array_equal(WIg, WIp), case
+        assert rel_err(Cg[s], Cp) <= MIXED_TOL, (case, s, rel_err(Cg[s], Cp))
+        assert rel_err(Ig[s], Ip) <= MIXED_TOL, (case, s, rel_err(Ig[s], Ip))
+
+
+def test_batch_linearity_and_permutation():
+    """the batch is linear in the skies and each sky's result does not depend
+    on its position in the batch or on its neighbours"""
+    kw = dict(nside=16, n_ant=8, n_epochs=16, segs=2, img_n=32, precision=1, nsky=18)
+    npix = 12 * 16 * 16
+    skies = _skies(18, npix, 55)
+    mix = skies.copy()
+    mix[17] = 2.0 * skies[3] + 0.5 * skies[9]
+    perm = np.arange(18)[::-1].copy()
+    out = {}
+    for name, S in (("base", skies), ("mix", mix), ("perm", skies[perm])):
+        with Pipeline(config(**kw)) as g:
+            g.set_sky(S)
+            g.run_all()
+            out[name] = g.finalize()
+    C, _, I, _ = out["base"]
+    Cm, _, Im, _ = out["mix"]
+    Cq, _, Iq, _ = out["perm"]
+    assert rel_err(Cm[17], 2.0 * C[3] + 0.5 * C[9]) <= MIXED_TOL
+    assert rel_err(Im[17], 2.0 * I[3] + 0.5 * I[9]) <= MIXED_TOL
+    assert np.array_equal(Cm[:17], C[:17])  # other skies are untouched bit for bit
+    assert rel_err(Cq, C[perm]) <= MIXED_TOL and rel_err(Iq, I[perm]) <= MIXED_TOL
+
+
+def test_batch_sky_errors():
+    kw = dict(nside=8, n_ant=4, n_epochs=4, segs=1, img_n=0, precision=1, nsky=3)
+    with Pipeline(config(**kw)) as g:
+        sk = np.ones((2, 12 * 64))
+        with pytest.raises(VissError):
+            g.set_sky(sk)  # two skies for a three-sky context
+        sk = np.ones((3, 12 * 64))
+        sk[1, 5] = np.nan
+        with pytest.raises(VissError):
+            g.set_sky(sk)
+        with pytest.raises(VissError):
+            g.viss_counts()

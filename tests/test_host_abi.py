@@ -73,7 +73,10 @@ def test_create_rejects_bad_configs():
     lib = V.load_library()
     ctx = C.c_void_p()
     for bad in (dict(tile_pix=300), dict(nside=0), dict(n_epochs=0), dict(blockage=2), dict(precision=5),
-                dict(recon_mode=3), dict(n_ant=1)):
+                dict(recon_mode=3), dict(n_ant=1),
+                # batched skies: mixed precision only, one member per group, at most 4096
+                dict(nsky=4), dict(nsky=4, precision=1, occ_mode=0), dict(nsky=4, precision=1, occ_mode=3),
+                dict(nsky=5000, precision=1)):
         cfg = V.default_config(1, **{**dict(nside=4, n_epochs=4, segs=1, img_n=0), **bad})
         assert lib.vw_create(C.byref(cfg), C.byref(ctx)) == A.VW_ERR_ARG, bad
         assert lib.vw_last_error(None)

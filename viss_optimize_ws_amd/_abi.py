@@ -57,6 +57,7 @@ class vw_config(C.Structure):
         ("class_budget_bytes", C.c_int64),
         ("serial_stages", C.c_int32),
         ("viss_form", C.c_int32),
+        ("nsky", C.c_int32),
     ]
 
 

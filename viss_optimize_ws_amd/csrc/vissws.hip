@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/vissws.h"
+#include "vw_batch.h"
 #include "vw_kernels.h"
 #include "vw_shared.h"
 
@@ -115,6 +116,20 @@ struct Ctx {
   int64_t nq_pad = 0;
   bool generic = false; /* recon needs per-row classes of the image tiles */
   DevInfo* d_info = nullptr;
+  /* sky batch (nsky > 1): packed sky operand, Viss chunk partials, packed
+   * group visibilities, recon partials of both output sets */
+  bool batch = false;
+  int64_t nsky = 1, nsky_pad = 16, nsg = 1;
+  int nbw = 1;                          /* 16-sky blocks per wave (1..4)         */
+  int64_t vb_target = 8192, rb_target = 16384; /* waves per launch (autotuned)  */
+  float* d_skyf = nullptr;
+  __bf16 *d_skyh = nullptr, *d_skyl = nullptr;
+  double2* d_bpart = nullptr;
+  int64_t bpart_cap = 0; /* double2 entries */
+  __bf16* d_gpk[4] = {};  /* re hi, re lo, im hi, im lo: [nq_pad / 32][nsky_pad][32] */
+  double *d_brpart = nullptr, *d_brpart2 = nullptr;
+  uint32_t *d_bwpart = nullptr, *d_bwpart2 = nullptr;
+  int64_t brpart_cap = 0, bwpart_cap = 0;
   double *d_Cacc = nullptr, *d_Gacc = nullptr;
   uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
   double *d_out = nullptr, *d_iout = nullptr;
@@ -308,6 +323,19 @@ int make_plan(Ctx* c, const vw_config* cfg) {
    * occ_mode 2 form (recon_seg_blockage_half_rep); occ_mode 0 and 3 average
    * several members per output point */
   c->generic = cfg->blockage && (cfg->occ_mode == 0 || cfg->occ_mode == 3);
+  c->nsky = cfg->nsky > 1 ? cfg->nsky : 1;
+  c->batch = c->nsky > 1;
+  if (c->batch) {
+    if (cfg->nsky > 4096) return c->fail(VW_ERR_ARG, "nsky above 4096");
+    if (cfg->precision != VW_PREC_MIXED) return c->fail(VW_ERR_ARG, "batched skies (nsky > 1) need the mixed precision");
+    if (c->generic)
+      return c->fail(VW_ERR_ARG, "batched skies need a recon rule with one member per group (occ_mode 1 or 2) or blockage 0");
+    c->vfact = false;
+    const int64_t b16 = (c->nsky + 15) / 16;
+    c->nbw = (int)std::min<int64_t>(4, b16);
+    c->nsg = (b16 + c->nbw - 1) / c->nbw;
+    c->nsky_pad = c->nsg * c->nbw * 16;
+  }
   if (c->max_nh > (int64_t)INT32_MAX / 4) return c->fail(VW_ERR_RANGE, "segment too long for 32-bit row indices");
   return VW_OK;
 }
@@ -400,6 +428,39 @@ int64_t recon_pass_tiles(const Ctx* c, int64_t ntile_all, int64_t kind_w) {
   return std::max<int64_t>(1, std::min<int64_t>(ntile_all, c->cls_cap / kind_w));
 }
 
+/* batched launch shapes.  Viss: a wave is 16 half rows x 16 nbw skies over a
+ * chunk of tiles, a workgroup 4 waves (64 rows) of one chunk; recon: a wave
+ * is 16 output points x 16 nbw skies over a range of groups.  The chunk
+ * counts follow a target of waves per launch, which the autotuner picks
+ * from VW_BATCH_TARGETS (vw_autotune). */
+#define VW_BATCH_MAX_CHUNKS 64
+constexpr int64_t VW_BATCH_TARGETS[] = {2048, 4096, 8192, 16384, 32768, 65536};
+struct VissBShape {
+  int64_t nrb, nrbw, tpc, nchunk, nblocks;
+};
+VissBShape vissb_shape(const Ctx* c, int64_t nh, int64_t nt, int64_t target) {
+  VissBShape s;
+  s.nrb = (nh + 15) / 16;
+  s.nrbw = (s.nrb + 3) / 4;
+  int64_t nchunk = (target + s.nrb * c->nsg - 1) / (s.nrb * c->nsg);
+  nchunk = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nchunk, nt), VW_BATCH_MAX_CHUNKS));
+  s.tpc = (nt + nchunk - 1) / nchunk;
+  s.nchunk = (nt + s.tpc - 1) / s.tpc;
+  s.nblocks = s.nchunk * s.nrbw * c->nsg;
+  return s;
+}
+struct ReconBShape {
+  int64_t pblk, pbw, nchunk, nblocks;
+};
+ReconBShape reconb_shape(const Ctx* c, int64_t np, int64_t target) {
+  ReconBShape s;
+  s.pblk = (np + 15) / 16;
+  s.pbw = (s.pblk + 3) / 4;
+  s.nchunk = std::max<int64_t>(1, std::min<int64_t>(VW_BATCH_MAX_CHUNKS, (target + s.pblk * c->nsg - 1) / (s.pblk * c->nsg)));
+  s.nblocks = s.nchunk * s.pbw * c->nsg;
+  return s;
+}
+
 int upload_day(Ctx* c) {
   /* lunar t1s windows (orbit_online.cuh:317-333) */
   if (c->cfg.geometry == VW_GEOM_LUNAR8) {
@@ -477,6 +538,46 @@ int join_recon(Ctx* c) {
   return VW_OK;
 }
 
+/* device buffers of the batched form, sized for every launch shape the
+ * autotuner may choose */
+int create_batch(Ctx* c) {
+  int rc;
+  const int64_t npix_pad = c->ntile * c->TP;
+  if ((rc = dalloc(c, &c->d_skyf, (size_t)(c->nsky * c->npix))) ||
+      (rc = dalloc(c, &c->d_skyh, (size_t)(npix_pad * c->nsky_pad))) ||
+      (rc = dalloc(c, &c->d_skyl, (size_t)(npix_pad * c->nsky_pad))))
+    return rc;
+  for (int k = 0; k < 4; ++k)
+    if ((rc = dalloc(c, &c->d_gpk[k], (size_t)(c->nq_pad * c->nsky_pad)))) return rc;
+  int64_t vch = 1;
+  const int64_t vlast = c->ntile - (c->ntile - 1) / c->vtpp * c->vtpp;
+  for (int k = 0; k < c->segs; ++k)
+    for (int64_t nt : {c->vtpp, vlast})
+      for (int64_t tg : VW_BATCH_TARGETS) vch = std::max(vch, vissb_shape(c, (int64_t)c->seg_tlen[k] * c->P, nt, tg).nchunk);
+  c->bpart_cap = vch * c->nsky_pad * c->nh_pad;
+  if ((rc = dalloc(c, &c->d_bpart, (size_t)c->bpart_cap))) return rc;
+  const int64_t kind_w = c->nq_pad;
+  int64_t rp = 1, wp = 1;
+  for (int img = 0; img < 2; ++img) {
+    const int64_t npts = img ? c->ngp : c->npix, nta = img ? c->ngt : c->ntile;
+    if (npts <= 0) continue;
+    const int64_t pp = recon_pass_tiles(c, nta, kind_w) * c->TP;
+    const int64_t last = npts - (npts - 1) / pp * pp;
+    for (int64_t n : {std::min(npts, pp), last})
+      for (int64_t tg : VW_BATCH_TARGETS) {
+        const int64_t nch = reconb_shape(c, n, tg).nchunk;
+        rp = std::max(rp, nch * c->nsky_pad * n);
+        wp = std::max(wp, nch * n);
+      }
+  }
+  c->brpart_cap = rp;
+  c->bwpart_cap = wp;
+  if ((rc = dalloc(c, &c->d_brpart, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart, (size_t)wp))) return rc;
+  if (c->ngp > 0 && ((rc = dalloc(c, &c->d_brpart2, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart2, (size_t)wp))))
+    return rc;
+  return VW_OK;
+}
+
 int create_device(Ctx* c) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return c->fail(VW_ERR_DEVICE, "no HIP device");
@@ -495,8 +596,8 @@ int create_device(Ctx* c) {
   const size_t np = (size_t)c->npix;
   if ((rc = dalloc(c, &c->d_l, np)) || (rc = dalloc(c, &c->d_m, np)) || (rc = dalloc(c, &c->d_n, np)) ||
       (rc = dalloc(c, &c->d_nm1, np)) || (rc = dalloc(c, &c->d_cone, 5 * (size_t)c->ntile)) ||
-      (rc = dalloc(c, &c->d_B, np)) || (rc = dalloc(c, &c->d_Bf, np)) || (rc = dalloc(c, &c->d_Cacc, np)) ||
-      (rc = dalloc(c, &c->d_Wacc, np)))
+      (rc = dalloc(c, &c->d_B, np)) || (rc = dalloc(c, &c->d_Bf, np)) ||
+      (rc = dalloc(c, &c->d_Cacc, np * (size_t)c->nsky)) || (rc = dalloc(c, &c->d_Wacc, np)))
     return rc;
   k_pix2lmn<<<blocks_for(c->npix), 256, 0, c->st>>>(c->nside, c->npix, c->d_l, c->d_m, c->d_n, c->d_nm1);
   KC();
@@ -507,7 +608,7 @@ int create_device(Ctx* c) {
     const size_t ng = (size_t)c->ngp;
     if ((rc = dalloc(c, &c->d_gidx, ng)) || (rc = dalloc(c, &c->d_gl, ng)) || (rc = dalloc(c, &c->d_gm, ng)) ||
         (rc = dalloc(c, &c->d_gn, ng)) || (rc = dalloc(c, &c->d_gcone, 5 * (size_t)c->ngt)) ||
-        (rc = dalloc(c, &c->d_Gacc, ng)) || (rc = dalloc(c, &c->d_GWacc, ng)))
+        (rc = dalloc(c, &c->d_Gacc, ng * (size_t)c->nsky)) || (rc = dalloc(c, &c->d_GWacc, ng)))
       return rc;
     HC(hipMemcpyAsync(c->d_gidx, c->gidx.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice, c->st));
     k_grid_points<<<blocks_for(c->ngp), 256, 0, c->st>>>(c->cfg.img_n, c->ngp, c->d_gidx, c->d_gl, c->d_gm, c->d_gn);
@@ -554,9 +655,10 @@ int create_device(Ctx* c) {
   }
   for (int b = 0; b < 2; ++b) {
     Ctx::SegSet& g = c->seg[b];
+    const size_t msk = mh * (size_t)c->nsky; /* batched: [nsky][nh] */
     if ((rc = dalloc(c, &g.rows, mh)) || (rc = dalloc(c, &g.rowt, (size_t)c->nh_pad)) ||
-        (rc = dalloc(c, &g.ant8, 8 * (size_t)c->max_tlen * c->A)) || (rc = dalloc(c, &g.vraw, mh)) ||
-        (rc = dalloc(c, &g.vh, mh)) || (rc = dalloc(c, &g.vfull, 2 * mh)))
+        (rc = dalloc(c, &g.ant8, 8 * (size_t)c->max_tlen * c->A)) || (rc = dalloc(c, &g.vraw, msk)) ||
+        (rc = dalloc(c, &g.vh, msk)) || (rc = dalloc(c, &g.vfull, 2 * msk)))
       return rc;
   }
   select_seg(c, 1);
@@ -566,8 +668,8 @@ int create_device(Ctx* c) {
       (rc = dalloc(c, &c->d_gkeys, 2 * mh + 1)) || (rc = dalloc(c, &c->d_idx_sorted, 2 * mh)) ||
       (rc = dalloc(c, &c->d_goff, 2 * mh + 2)) || (rc = dalloc(c, &c->d_rank, 2 * mh)) ||
       (rc = dalloc(c, &c->d_info, 1)) ||
-      (rc = dalloc(c, &c->d_out, (size_t)c->npix)) || (rc = dalloc(c, &c->d_wout, (size_t)c->npix)) ||
-      (rc = dalloc(c, &c->d_iout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))) ||
+      (rc = dalloc(c, &c->d_out, (size_t)c->npix * c->nsky)) || (rc = dalloc(c, &c->d_wout, (size_t)c->npix)) ||
+      (rc = dalloc(c, &c->d_iout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n) * c->nsky)) ||
       (rc = dalloc(c, &c->d_iwout, (size_t)std::max<int64_t>(1, (int64_t)c->cfg.img_n * c->cfg.img_n))))
     return rc;
   {
@@ -611,11 +713,12 @@ int create_device(Ctx* c) {
                        (rc = dalloc(c, &c->d_rwpart2, (size_t)c->rpart_cap))))
       return rc;
   }
+  if (c->batch && (rc = create_batch(c))) return rc;
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
-  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np, c->st));
+  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * np * c->nsky, c->st));
   HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * np, c->st));
   if (c->ngp) {
-    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp * c->nsky, c->st));
     HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
   }
   if ((rc = upload_day(c))) return rc;
@@ -716,8 +819,40 @@ int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
   return VW_OK;
 }
 
+/* one tile pass of the batched Viss: classes, the product kernel, the fold */
+int enq_viss_batch(Ctx* c, int64_t tile0, int64_t nt) {
+  int rc;
+  if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
+  const VissBShape s = vissb_shape(c, c->nh, nt, c->vb_target);
+  if (s.nchunk * c->nsky_pad * c->nh_pad > c->bpart_cap) return c->fail(VW_ERR_RANGE, "internal: batch partial buffer too small");
+  VissBArgs a{};
+  a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1;
+  a.skyh = (const uint4*)c->d_skyh; a.skyl = (const uint4*)c->d_skyl;
+  a.npix = c->npix; a.nsky_pad = c->nsky_pad;
+  a.tile0 = tile0; a.ntile = nt; a.TP = c->TP;
+  a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = c->d_cls_v;
+  a.nh = c->nh; a.nh_pad = c->nh_pad;
+  a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.nrbw = s.nrbw;
+  a.part = c->d_bpart;
+  ev_begin(c, ST_VISSK);
+  const unsigned nb = (unsigned)s.nblocks;
+  switch (c->nbw) {
+    case 1: k_vissb<1><<<nb, 256, 0, c->st>>>(a); break;
+    case 2: k_vissb<2><<<nb, 256, 0, c->st>>>(a); break;
+    case 3: k_vissb<3><<<nb, 256, 0, c->st>>>(a); break;
+    default: k_vissb<4><<<nb, 256, 0, c->st>>>(a); break;
+  }
+  KC();
+  ev_end(c, ST_VISSK);
+  k_vissb_fold<<<blocks_for(c->nsky * c->nh), 256, 0, c->st>>>(c->d_bpart, s.nchunk, c->nsky, c->nsky_pad, c->nh,
+                                                               c->nh_pad, tile0 > 0 ? 1 : 0, c->d_vraw);
+  KC();
+  return VW_OK;
+}
+
 int enq_viss(Ctx* c, bool count) {
   if (!c->have_seg || !c->have_sky) return c->fail(VW_ERR_STATE, "vw_viss_half needs vw_set_sky and vw_gen_segment");
+  if (count && c->batch) return c->fail(VW_ERR_ARG, "per-row counts come from the one-sky form (nsky = 1)");
   if (count && !c->d_cpart) {
     int rc2;
     if ((rc2 = dalloc(c, &c->d_cpart, (size_t)c->nchunk_max * c->nh_pad)) || (rc2 = dalloc(c, &c->d_counts, (size_t)c->max_nh)))
@@ -728,6 +863,10 @@ int enq_viss(Ctx* c, bool count) {
   ev_begin(c, ST_VISS);
   for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
+    if (c->batch) {
+      if ((rc = enq_viss_batch(c, tile0, nt))) return rc;
+      continue;
+    }
     if (c->vfact && !count) {
       if ((rc = enq_viss_fact(c, tile0, nt))) return rc;
       continue;
@@ -769,7 +908,11 @@ int enq_viss(Ctx* c, bool count) {
 int enq_conj(Ctx* c) {
   if (!c->have_viss) return c->fail(VW_ERR_STATE, "vw_conj_fill needs vw_viss_half");
   ev_begin(c, ST_CONJ);
-  k_conj_fill<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_vraw, c->d_rows, c->nh, c->P, c->d_vh, c->d_vfull);
+  if (c->batch)
+    k_conj_fill_b<<<blocks_for(c->nsky * c->nh), 256, 0, c->st>>>(c->d_vraw, c->d_rows, c->nh, c->P, c->nsky, c->d_vh,
+                                                                  c->d_vfull);
+  else
+    k_conj_fill<<<blocks_for(c->nh), 256, 0, c->st>>>(c->d_vraw, c->d_rows, c->nh, c->P, c->d_vh, c->d_vfull);
   KC();
   ev_end(c, ST_CONJ);
   HC(hipEventRecord(c->ev_conj, c->st));
@@ -815,6 +958,12 @@ int enq_group(Ctx* c) {
                                                   c->cfg.precision == VW_PREC_MIXED ? 1.0 : VW_TAB_SCALE, c->d_info,
                                                   c->gs);
   KC();
+  if (c->batch) {
+    k_group_pack<<<blocks_for(c->nq_pad * c->nsky_pad), 256, 0, c->stB>>>(
+        c->d_goff, c->d_idx_sorted, c->d_vh, c->nh, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode, c->d_info,
+        c->nq_pad, c->nsky, c->nsky_pad, c->d_gpk[0], c->d_gpk[1], c->d_gpk[2], c->d_gpk[3]);
+    KC();
+  }
   ev_end(c, ST_GROUP, c->stB);
   c->have_groups = true;
   return VW_OK;
@@ -826,7 +975,63 @@ void launch_recon(const ReconArgs& a, hipStream_t st) {
   k_recon<RECON_PPL, MIXED, KIND><<<blocks_for(waves, 4), 256, 0, st>>>(a);
 }
 
+/* batched recon of one output set: per tile pass the (tile, group) classes,
+ * the product kernel and the fold into the per-sky accumulators */
+int enq_recon_set_batch(Ctx* c, bool img) {
+  hipStream_t st = img ? c->st2 : c->stB;
+  uint8_t* cls = img ? c->d_cls2 : c->d_cls;
+  uint8_t* bits = img ? c->d_bits2 : c->d_bits;
+  const bool rep = c->cfg.blockage != 0;
+  const int64_t npts_all = img ? c->ngp : c->npix;
+  const int64_t ntile_all = img ? c->ngt : c->ntile;
+  const double* cone = img ? c->d_gcone : c->d_cone;
+  const int64_t otpp = recon_pass_tiles(c, ntile_all, c->nq_pad);
+  for (int64_t tile0 = 0; tile0 < ntile_all; tile0 += otpp) {
+    const int64_t nt = std::min(otpp, ntile_all - tile0);
+    const int64_t pt0 = tile0 * c->TP;
+    const int64_t np = std::min(nt * c->TP, npts_all - pt0);
+    if (rep) {
+      int rc = enq_classes(c, cone, ntile_all, tile0, nt, true, bits, cls, st);
+      if (rc) return rc;
+    }
+    const ReconBShape s = reconb_shape(c, np, c->rb_target);
+    if (s.nchunk * c->nsky_pad * np > c->brpart_cap || s.nchunk * np > c->bwpart_cap)
+      return c->fail(VW_ERR_RANGE, "internal: batch recon partial buffer too small");
+    ReconBArgs a{};
+    a.l = img ? c->d_gl : c->d_l;
+    a.m = img ? c->d_gm : c->d_m;
+    a.n = img ? c->d_gn : c->d_n;
+    a.pt0 = pt0; a.npts = np; a.TP = c->TP;
+    a.gcls = cls; a.nq_pad = c->nq_pad;
+    a.rec = c->gs.rec; a.gant = c->gs.ant; a.info = c->d_info;
+    a.vrh = (const uint4*)c->d_gpk[0]; a.vrl = (const uint4*)c->d_gpk[1];
+    a.vih = (const uint4*)c->d_gpk[2]; a.vil = (const uint4*)c->d_gpk[3];
+    a.nsky_pad = c->nsky_pad;
+    a.rep = rep ? 1 : 0;
+    a.nchunk = s.nchunk; a.pblk = s.pblk; a.pbw = s.pbw;
+    a.part = img ? c->d_brpart2 : c->d_brpart;
+    a.wpart = img ? c->d_bwpart2 : c->d_bwpart;
+    a.pstride = np;
+    ev_begin(c, ST_RECONK, st);
+    const unsigned nb = (unsigned)s.nblocks;
+    switch (c->nbw) {
+      case 1: k_reconb<1><<<nb, 256, 0, st>>>(a); break;
+      case 2: k_reconb<2><<<nb, 256, 0, st>>>(a); break;
+      case 3: k_reconb<3><<<nb, 256, 0, st>>>(a); break;
+      default: k_reconb<4><<<nb, 256, 0, st>>>(a); break;
+    }
+    KC();
+    ev_end(c, ST_RECONK, st);
+    k_reconb_fold<<<blocks_for(c->nsky * np), 256, 0, st>>>(a.part, a.wpart, s.nchunk, c->nsky, c->nsky_pad, np, pt0,
+                                                             np, npts_all, c->cfg.blockage, img ? c->d_Gacc : c->d_Cacc,
+                                                             img ? c->d_GWacc : c->d_Wacc);
+    KC();
+  }
+  return VW_OK;
+}
+
 int enq_recon_set(Ctx* c, bool img) {
+  if (c->batch) return enq_recon_set_batch(c, img);
   /* each output set has its own stream and scratch */
   hipStream_t st = img ? c->st2 : c->stB;
   uint8_t* cls = img ? c->d_cls2 : c->d_cls;
@@ -957,6 +1162,7 @@ VW_API int vw_config_init(vw_config* cfg, int32_t geometry) {
   cfg->tile_pix = 256;
   cfg->precision = VW_PREC_PLAIN;
   cfg->healpix_out = 1;
+  cfg->nsky = 1;
   cfg->gm_km3_s2 = 398600.4418;
   if (geometry == VW_GEOM_LUNAR8) {
     cfg->nside = 4096;
@@ -1071,7 +1277,23 @@ VW_API int vw_set_sky(void* p, const double* B, int64_t npix, int32_t nsky) {
   Ctx* c = as_ctx(p);
   if (!c || !B) return VW_ERR_ARG;
   if (npix != c->npix) return c->fail(VW_ERR_ARG, "sky has " + std::to_string(npix) + " pixels, expected " + std::to_string(c->npix));
-  if (nsky != 1) return c->fail(VW_ERR_ARG, "nsky must be 1");
+  if (nsky != c->nsky) return c->fail(VW_ERR_ARG, "nsky must equal the configured nsky (" + std::to_string(c->nsky) + ")");
+  if (c->batch) {
+    /* FP32 of B * pix_area (main_recon_online.cu:106-110), then hi / lo BF16 */
+    std::vector<float> f((size_t)(npix * nsky));
+    for (int64_t i = 0; i < npix * nsky; ++i) {
+      if (!std::isfinite(B[i])) return c->fail(VW_ERR_ARG, "batched skies must be finite");
+      f[i] = (float)(B[i] * c->pix_area);
+    }
+    HC(hipMemcpyAsync(c->d_skyf, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, c->st));
+    const int64_t npix_pad = c->ntile * c->TP;
+    k_sky_pack<<<blocks_for(npix_pad * c->nsky_pad), 256, 0, c->st>>>(c->d_skyf, c->nsky, npix, npix_pad, c->nsky_pad,
+                                                                       c->d_skyh, c->d_skyl);
+    KC();
+    HC(hipStreamSynchronize(c->st));
+    c->have_sky = true;
+    return VW_OK;
+  }
   std::vector<double> b((size_t)npix);
   std::vector<float> bf((size_t)npix);
   for (int64_t i = 0; i < npix; ++i) {
@@ -1122,7 +1344,7 @@ VW_API int vw_viss_half(void* p, vw_c128* out) {
   if (!c) return VW_ERR_ARG;
   int rc = enq_viss(c, false);
   if (rc) return rc;
-  if (out) HC(hipMemcpyAsync(out, c->d_vraw, sizeof(vw_c128) * c->nh, hipMemcpyDeviceToHost, c->st));
+  if (out) HC(hipMemcpyAsync(out, c->d_vraw, sizeof(vw_c128) * c->nh * c->nsky, hipMemcpyDeviceToHost, c->st));
   return sync(c);
 }
 
@@ -1131,8 +1353,8 @@ VW_API int vw_conj_fill(void* p, vw_c128* oh, vw_c128* of) {
   if (!c) return VW_ERR_ARG;
   int rc = enq_conj(c);
   if (rc) return rc;
-  if (oh) HC(hipMemcpyAsync(oh, c->d_vh, sizeof(vw_c128) * c->nh, hipMemcpyDeviceToHost, c->st));
-  if (of) HC(hipMemcpyAsync(of, c->d_vfull, sizeof(vw_c128) * 2 * c->nh, hipMemcpyDeviceToHost, c->st));
+  if (oh) HC(hipMemcpyAsync(oh, c->d_vh, sizeof(vw_c128) * c->nh * c->nsky, hipMemcpyDeviceToHost, c->st));
+  if (of) HC(hipMemcpyAsync(of, c->d_vfull, sizeof(vw_c128) * 2 * c->nh * c->nsky, hipMemcpyDeviceToHost, c->st));
   return sync(c);
 }
 
@@ -1168,19 +1390,32 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
     k_normalize<<<blocks_for(c->npix), 256, 0, c->st>>>(c->d_Cacc, c->d_Wacc, c->npix, c->cfg.blockage, c->d_info,
                                                         nullptr, c->d_out, c->d_wout);
     KC();
+    if (c->batch) {
+      k_normalize_b<<<blocks_for(c->npix * c->nsky), 256, 0, c->st>>>(c->d_Cacc, c->d_Wacc, c->npix, c->nsky,
+                                                                        c->cfg.blockage, c->d_info, nullptr, c->npix,
+                                                                        c->d_out);
+      KC();
+    }
   }
   if (c->ngp) {
     int64_t NN = (int64_t)c->cfg.img_n * c->cfg.img_n;
-    HC(hipMemsetAsync(c->d_iout, 0, sizeof(double) * NN, c->st));
+    HC(hipMemsetAsync(c->d_iout, 0, sizeof(double) * NN * c->nsky, c->st));
     HC(hipMemsetAsync(c->d_iwout, 0, sizeof(uint32_t) * NN, c->st));
     k_normalize<<<blocks_for(c->ngp), 256, 0, c->st>>>(c->d_Gacc, c->d_GWacc, c->ngp, c->cfg.blockage, c->d_info,
                                                        c->d_gidx, c->d_iout, c->d_iwout);
     KC();
+    if (c->batch) {
+      k_normalize_b<<<blocks_for(c->ngp * c->nsky), 256, 0, c->st>>>(c->d_Gacc, c->d_GWacc, c->ngp, c->nsky,
+                                                                       c->cfg.blockage, c->d_info, c->d_gidx, NN,
+                                                                       c->d_iout);
+      KC();
+    }
   }
   ev_end(c, ST_FIN);
   if (C) {
-    if (c->cfg.healpix_out) HC(hipMemcpyAsync(C, c->d_out, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
-    else std::memset(C, 0, sizeof(double) * c->npix);
+    if (c->cfg.healpix_out)
+      HC(hipMemcpyAsync(C, c->d_out, sizeof(double) * c->npix * c->nsky, hipMemcpyDeviceToHost, c->st));
+    else std::memset(C, 0, sizeof(double) * c->npix * c->nsky);
   }
   if (W) {
     if (c->cfg.healpix_out) HC(hipMemcpyAsync(W, c->d_wout, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
@@ -1188,7 +1423,7 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
   }
   if (c->ngp) {
     int64_t NN = (int64_t)c->cfg.img_n * c->cfg.img_n;
-    if (img) HC(hipMemcpyAsync(img, c->d_iout, sizeof(double) * NN, hipMemcpyDeviceToHost, c->st));
+    if (img) HC(hipMemcpyAsync(img, c->d_iout, sizeof(double) * NN * c->nsky, hipMemcpyDeviceToHost, c->st));
     if (Wimg) HC(hipMemcpyAsync(Wimg, c->d_iwout, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost, c->st));
   }
   return sync(c);
@@ -1198,10 +1433,10 @@ VW_API int vw_reset_accumulators(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
   if (int rj = join_recon(c)) return rj;
-  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
+  HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix * c->nsky, c->st));
   HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * c->npix, c->st));
   if (c->ngp) {
-    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp * c->nsky, c->st));
     HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
@@ -1212,9 +1447,9 @@ VW_API int vw_get_accumulators(void* p, double* C, uint32_t* W, double* G, uint3
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
   if (int rj = join_recon(c)) return rj;
-  if (C) HC(hipMemcpyAsync(C, c->d_Cacc, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
+  if (C) HC(hipMemcpyAsync(C, c->d_Cacc, sizeof(double) * c->npix * c->nsky, hipMemcpyDeviceToHost, c->st));
   if (W) HC(hipMemcpyAsync(W, c->d_Wacc, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
-  if (c->ngp && G) HC(hipMemcpyAsync(G, c->d_Gacc, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->st));
+  if (c->ngp && G) HC(hipMemcpyAsync(G, c->d_Gacc, sizeof(double) * c->ngp * c->nsky, hipMemcpyDeviceToHost, c->st));
   if (c->ngp && GW) HC(hipMemcpyAsync(GW, c->d_GWacc, sizeof(uint32_t) * c->ngp, hipMemcpyDeviceToHost, c->st));
   if (wtotal)
     HC(hipMemcpyAsync(wtotal, (char*)c->d_info + offsetof(DevInfo, wtotal), sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -1227,13 +1462,13 @@ VW_API int vw_set_accumulators(void* p, const double* C, const uint32_t* W, cons
   Ctx* c = as_ctx(p);
   if (!c || wtotal < 0) return VW_ERR_ARG;
   if (int rj = join_recon(c)) return rj;
-  if (C) HC(hipMemcpyAsync(c->d_Cacc, C, sizeof(double) * c->npix, hipMemcpyHostToDevice, c->st));
-  else HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix, c->st));
+  if (C) HC(hipMemcpyAsync(c->d_Cacc, C, sizeof(double) * c->npix * c->nsky, hipMemcpyHostToDevice, c->st));
+  else HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix * c->nsky, c->st));
   if (W) HC(hipMemcpyAsync(c->d_Wacc, W, sizeof(uint32_t) * c->npix, hipMemcpyHostToDevice, c->st));
   else HC(hipMemsetAsync(c->d_Wacc, 0, sizeof(uint32_t) * c->npix, c->st));
   if (c->ngp) {
-    if (G) HC(hipMemcpyAsync(c->d_Gacc, G, sizeof(double) * c->ngp, hipMemcpyHostToDevice, c->st));
-    else HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp, c->st));
+    if (G) HC(hipMemcpyAsync(c->d_Gacc, G, sizeof(double) * c->ngp * c->nsky, hipMemcpyHostToDevice, c->st));
+    else HC(hipMemsetAsync(c->d_Gacc, 0, sizeof(double) * c->ngp * c->nsky, c->st));
     if (GW) HC(hipMemcpyAsync(c->d_GWacc, GW, sizeof(uint32_t) * c->ngp, hipMemcpyHostToDevice, c->st));
     else HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
   }

@@ -1,0 +1,504 @@
+/*
+ * vw_batch.h -- sky batches: many sky realisations through one geometry.
+ *
+ * The skies of a batch share antennas, epochs, the segment schedule and so
+ * the rows, tile classes, w-plane fit, uv cells and groups of every segment.
+ * Both sums then become products of a phasor matrix that depends only on
+ * the geometry with a matrix that holds one column per sky:
+ *
+ *   Viss   V[h][s] = sum_p  M[h][p] exp(-j 2 pi x_hp) I[p][s]
+ *          (rows x pixels) x (pixels x skies)
+ *   recon  C[q][s] = sum_g  M[q][g] Re(exp(+j 2 pi y_qg) V_g[s])
+ *          (points x groups) x (groups x skies)
+ *
+ * The phasor tiles are made on the fly (FP64 phase and its reduction mod 1
+ * turn, hardware FP32 sin/cos, the occlusion mask as zeros) and never stored;
+ * the products run on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with
+ * both operands split x = hi + lo into two BF16 values and three products
+ * hi*hi + hi*lo + lo*hi per real multiply-add (about 2^-16 relative per
+ * term), FP32 accumulation over one 256-pixel sub-tile or 256-group block,
+ * then FP64 sums across blocks, chunks and passes.
+ *
+ * Operand layout of v_mfma_f32_16x16x32_bf16 (wave64), as used here:
+ *   A (16 x 32, rows x k):  lane l holds row l % 16, k = 8 (l / 16) + i, i < 8
+ *   B (32 x 16, k x cols):  lane l holds column l % 16, the same k
+ *   D (16 x 16, f32):       lane l holds D[4 (l / 16) + i][l % 16], i < 4
+ * A and B use the same k assignment, so any permutation of k inside the
+ * instruction leaves the product unchanged.
+ *
+ * Packed operand matrices (sky, group visibilities) are stored as
+ * [k / 32][nsky_pad][32] BF16, so a lane's 8 values are one 16-byte load.
+ */
+#ifndef VW_BATCH_H
+#define VW_BATCH_H
+
+#include "vw_kernels.h"
+
+typedef __bf16 vw_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float vw_f32x4b __attribute__((ext_vector_type(4)));
+
+/* x = hi + lo: hi keeps the upper 16 bits of x (exact BF16), lo = bf16(x - hi)
+ * rounded to nearest even; x - hi is exact in FP32 and |x - hi| < 2^-7 |x|,
+ * so hi + lo is within 2^-16 |x|.  Per pair of values: two masks, two
+ * subtractions, one byte permute for the hi pair, one conversion for the lo
+ * pair. */
+__device__ __forceinline__ void vw_split8(const float (&x)[8], vw_bf16x8& hi, vw_bf16x8& lo) {
+  uint32_t h[4];
+  float l[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t a = __builtin_bit_cast(uint32_t, x[2 * j]) & 0xffff0000u;
+    const uint32_t b = __builtin_bit_cast(uint32_t, x[2 * j + 1]) & 0xffff0000u;
+    l[2 * j] = x[2 * j] - __builtin_bit_cast(float, a);
+    l[2 * j + 1] = x[2 * j + 1] - __builtin_bit_cast(float, b);
+    h[j] = __builtin_amdgcn_perm(b, a, 0x07060302u); /* (b.hi << 16) | a.hi */
+  }
+  hi = __builtin_bit_cast(vw_bf16x8, h);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) lo[i] = (__bf16)l[i];
+}
+
+__device__ __forceinline__ vw_bf16x8 vw_ld_bf16x8(const uint4* p) {
+  return __builtin_bit_cast(vw_bf16x8, *p);
+}
+
+__device__ __forceinline__ vw_f32x4b vw_mfma_bf16(vw_bf16x8 a, vw_bf16x8 b, vw_f32x4b c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+/* three-product split multiply-accumulate: (ah + al)(bh + bl) without al*bl */
+__device__ __forceinline__ vw_f32x4b vw_mac3(vw_bf16x8 ah, vw_bf16x8 al, vw_bf16x8 bh, vw_bf16x8 bl,
+                                             vw_f32x4b c) {
+  c = vw_mfma_bf16(al, bh, c);
+  c = vw_mfma_bf16(ah, bl, c);
+  return vw_mfma_bf16(ah, bh, c);
+}
+
+/* ------------------------------------------------------------------ */
+/* packing                                                              */
+/* ------------------------------------------------------------------ */
+
+/* sky batch [nsky][npix] FP32 (already scaled by the pixel area) ->
+ * hi / lo [npix_pad / 32][nsky_pad][32]; zero for padding pixels and skies */
+__global__ void k_sky_pack(const float* __restrict__ sky, int64_t nsky, int64_t npix, int64_t npix_pad,
+                           int64_t nsky_pad, __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npix_pad * nsky_pad) return;
+  const int64_t kb = k / (nsky_pad * 32);
+  const int64_t r = k - kb * nsky_pad * 32;
+  const int64_t s = r / 32, j = r - s * 32;
+  const int64_t p = kb * 32 + j;
+  const float x = (s < nsky && p < npix) ? sky[s * npix + p] : 0.0f;
+  const __bf16 h = (__bf16)x;
+  hi[k] = h;
+  lo[k] = (__bf16)(x - (float)h);
+}
+
+/* ------------------------------------------------------------------ */
+/* Viss                                                                 */
+/* ------------------------------------------------------------------ */
+struct VissBArgs {
+  const double *l, *m, *n, *nm1;
+  const uint4 *skyh, *skyl; /* packed sky, 16-byte units                        */
+  int64_t npix, nsky_pad;
+  int64_t tile0, ntile, TP;  /* tiles of this pass                              */
+  const RowRec* rows;
+  const double* ant8;
+  const uint8_t* cls;        /* [ntile][nh_pad] pass-relative class bytes         */
+  int64_t nh, nh_pad;
+  int64_t tpc, nchunk, nrb, nrbw; /* tiles per chunk, chunks, 16-row blocks, blocks of 4 */
+  double2* part;             /* [nchunk][nsky_pad][nh_pad]                        */
+};
+
+/* one wave: 16 half rows x (16 NB) skies over the tiles of one chunk.  A
+ * workgroup is four waves (64 consecutive rows) on the same pixels, which it
+ * stages through LDS one 256-pixel sub-tile at a time. */
+template <int NB>
+__global__ __launch_bounds__(256) void k_vissb(VissBArgs a) {
+  __shared__ double px[4][256]; /* l, m, n - 1, n of one sub-tile */
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t rbw = vb % a.nrbw;
+  const int64_t rest = vb / a.nrbw;
+  const int64_t chunk = rest % a.nchunk;
+  const int64_t sg = rest / a.nchunk;
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t rb = rbw * 4 + wave;
+  const bool wave_on = rb < a.nrb; /* uniform; every wave still joins the barriers */
+  const int64_t h = rb * 16 + r;   /* < nh_pad: nrb * 16 <= nh_pad */
+  const int64_t hc = h < a.nh ? h : a.nh - 1;
+  const RowRec rec = a.rows[hc];
+  const int64_t t0 = a.tile0 + chunk * a.tpc;
+  const int64_t tend = a.tile0 + a.ntile;
+  const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
+  const int64_t nsub = a.TP / 256;
+
+  vw_f32x4b accR[NB], accI[NB];
+  double sumR[NB][4], sumI[NB][4];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+    accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { sumR[nb][i] = 0.0; sumI[nb][i] = 0.0; }
+  }
+
+  for (int64_t st = t0 * nsub; st < t1 * nsub; ++st) {
+    const int64_t tile = st / nsub;
+    const int64_t pbase = st * 256;
+    __syncthreads(); /* the previous sub-tile is consumed */
+    {
+      const int64_t p = pbase + threadIdx.x;
+      const bool okp = p < a.npix;
+      const int64_t pp = okp ? p : 0;
+      px[0][threadIdx.x] = okp ? a.l[pp] : 0.0;
+      px[1][threadIdx.x] = okp ? a.m[pp] : 0.0;
+      px[2][threadIdx.x] = okp ? a.nm1[pp] : 0.0;
+      px[3][threadIdx.x] = okp ? a.n[pp] : 0.0;
+    }
+    __syncthreads();
+    if (!wave_on) continue;
+    /* rows past nh carry the hidden class in the table */
+    const int cls = (int)a.cls[(tile - a.tile0) * a.nh_pad + h];
+    if (__ballot(cls != VW_CLS_HIDDEN) == 0) continue;
+    const bool any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
+    double ax = 0, ay = 0, az = 0, ai = 0, ac = 0, bx = 0, by = 0, bz = 0, bi = 0, bc = 0;
+    if (any_mixed && cls == VW_CLS_MIXED) {
+      const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+      const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+      ax = qa[0]; ay = qa[1]; az = qa[2]; ai = qa[3]; ac = qa[7];
+      bx = qb[0]; by = qb[1]; bz = qb[2]; bi = qb[3]; bc = qb[7];
+    }
+    const int64_t kb0 = pbase >> 5;
+    auto load_b = [&](int ks, vw_bf16x8 (&bh)[NB], vw_bf16x8 (&bl)[NB]) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int64_t off = ((kb0 + ks) * a.nsky_pad + (sg * NB + nb) * 16 + r) * 4 + g;
+        bh[nb] = vw_ld_bf16x8(a.skyh + off);
+        bl[nb] = vw_ld_bf16x8(a.skyl + off);
+      }
+    };
+    auto step = [&](int ks, const vw_bf16x8 (&bh)[NB], const vw_bf16x8 (&bl)[NB]) {
+      float cv[8], sv[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = ks * 32 + g * 8 + i;
+        const double pl = px[0][k], pm = px[1][k], pz = px[2][k];
+        const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl, pm, pz);
+        const float f = (float)__builtin_amdgcn_fract(x);
+        const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+        bool on = cls == VW_CLS_VISIBLE;
+        if (any_mixed && cls == VW_CLS_MIXED) {
+          const double pn = px[3][k];
+          on = vw_visible(pl, pm, pn, ax, ay, az, ai, ac) && vw_visible(pl, pm, pn, bx, by, bz, bi, bc);
+        }
+        cv[i] = on ? cs : 0.0f;
+        sv[i] = on ? sn : 0.0f;
+      }
+      vw_bf16x8 ach, acl, ash, asl;
+      vw_split8(cv, ach, acl);
+      vw_split8(sv, ash, asl);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        accR[nb] = vw_mac3(ach, acl, bh[nb], bl[nb], accR[nb]);
+        accI[nb] = vw_mac3(ash, asl, bh[nb], bl[nb], accI[nb]);
+      }
+    };
+#pragma unroll 1
+    for (int ks = 0; ks < 8; ++ks) {
+      vw_bf16x8 bh[NB], bl[NB];
+      load_b(ks, bh, bl);
+      step(ks, bh, bl);
+    }
+    /* FP64 sums across sub-tiles */
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        sumR[nb][i] += (double)accR[nb][i];
+        sumI[nb][i] += (double)accI[nb][i];
+      }
+      accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+      accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+  if (!wave_on) return;
+  /* D[4 g + i][r]: half row rb * 16 + 4 g + i, sky column r */
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int64_t sky = (sg * NB + nb) * 16 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t hh = rb * 16 + 4 * g + i;
+      if (hh < a.nh) a.part[(chunk * a.nsky_pad + sky) * a.nh_pad + hh] = make_double2(sumR[nb][i], -sumI[nb][i]);
+    }
+  }
+}
+
+/* chunk partials -> raw half sums [nsky][nh], chunks in ascending order;
+ * passes after the first add to the running sums */
+__global__ void k_vissb_fold(const double2* __restrict__ part, int64_t nchunk, int64_t nsky, int64_t nsky_pad,
+                             int64_t nh, int64_t nh_pad, int accumulate, double2* __restrict__ vraw) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsky * nh) return;
+  const int64_t s = k / nh, h = k - s * nh;
+  double re = 0.0, im = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    const double2 v = part[(c * nsky_pad + s) * nh_pad + h];
+    re += v.x;
+    im += v.y;
+  }
+  if (accumulate) {
+    re += vraw[k].x;
+    im += vraw[k].y;
+  }
+  vraw[k] = make_double2(re, im);
+}
+
+/* phase correction and conjugate completion of every sky [nsky][nh] ->
+ * [nsky][nh] and [nsky][2 nh] (common.hpp:255-277) */
+__global__ void k_conj_fill_b(const double2* __restrict__ vraw, const RowRec* __restrict__ rows, int64_t nh, int P,
+                              int64_t nsky, double2* __restrict__ vh, double2* __restrict__ vfull) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsky * nh) return;
+  const int64_t s = k / nh, h = k - s * nh;
+  double re, im;
+  vw_phase_correct(vraw[k].x, vraw[k].y, rows[h].w, &re, &im);
+  vh[k] = make_double2(re, im);
+  const int64_t g = h / P, j = h - g * P;
+  double2* vf = vfull + s * 2 * nh;
+  vf[2 * P * g + j] = make_double2(re, im);
+  vf[2 * P * g + P + j] = make_double2(re, -im);
+}
+
+/* ------------------------------------------------------------------ */
+/* reconstruction                                                       */
+/* ------------------------------------------------------------------ */
+
+/* group visibilities of every sky -> packed real and imaginary parts,
+ * [nq_pad / 32][nsky_pad][32] hi / lo; zero outside [first_nz, nuniq) and
+ * for padding skies.  blockage 1: the representative's value (occ_mode 1
+ * first member, else the median member, signed for rows of the conjugate
+ * half); blockage 0: the group mean over members in sorted order. */
+__global__ void k_group_pack(const int32_t* __restrict__ goff, const int32_t* __restrict__ idx_sorted,
+                             const double2* __restrict__ vh, int64_t nh, int P, int full, int blockage, int occ_mode,
+                             const DevInfo* __restrict__ info, int64_t nq_pad, int64_t nsky, int64_t nsky_pad,
+                             __bf16* __restrict__ rh, __bf16* __restrict__ rl, __bf16* __restrict__ ih,
+                             __bf16* __restrict__ il) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq_pad * nsky_pad) return;
+  const int64_t qb = k / (nsky_pad * 32);
+  const int64_t rr = k - qb * nsky_pad * 32;
+  const int64_t s = rr / 32, j = rr - s * 32;
+  const int64_t q = qb * 32 + j;
+  float vr = 0.f, vi = 0.f;
+  if (s < nsky && q >= info->first_nz && q < info->nuniq) {
+    const int32_t s0 = goff[q], s1 = goff[q + 1];
+    const double2* v = vh + s * nh;
+    if (blockage) {
+      bool neg;
+      const int64_t hr = vw_row_half(idx_sorted[occ_mode == 1 ? s0 : s0 + (s1 - s0) / 2], P, full != 0, &neg);
+      vr = (float)v[hr].x;
+      vi = (float)(neg ? -v[hr].y : v[hr].y);
+    } else {
+      double sr = 0.0, si = 0.0;
+      for (int32_t t = s0; t < s1; ++t) {
+        bool ng;
+        const int64_t hh = vw_row_half(idx_sorted[t], P, full != 0, &ng);
+        sr = sr + v[hh].x;
+        si = si + (ng ? -v[hh].y : v[hh].y);
+      }
+      vr = (float)(sr / (double)(s1 - s0));
+      vi = (float)(si / (double)(s1 - s0));
+    }
+  }
+  const __bf16 a = (__bf16)vr, b = (__bf16)vi;
+  rh[k] = a;
+  rl[k] = (__bf16)(vr - (float)a);
+  ih[k] = b;
+  il[k] = (__bf16)(vi - (float)b);
+}
+
+struct ReconBArgs {
+  const double *l, *m, *n;
+  int64_t pt0, npts, TP;      /* points [pt0, pt0 + npts) of this pass            */
+  const uint8_t* gcls;        /* [tiles of the pass][nq_pad] (representative kind) */
+  int64_t nq_pad;
+  const GRec* rec;            /* cell centres (wavelengths)                        */
+  const double* gant;         /* [q][10] representative antenna frames             */
+  const DevInfo* info;
+  const uint4 *vrh, *vrl, *vih, *vil; /* packed group visibilities, 16-byte units */
+  int64_t nsky_pad;
+  int rep;                    /* 1: representative classes; 0: every group counts  */
+  int64_t nchunk, pblk, pbw;  /* group ranges, 16-point blocks, blocks of 4        */
+  double* part;               /* [nchunk][nsky_pad][pstride]                       */
+  uint32_t* wpart;            /* [nchunk][pstride]                                 */
+  int64_t pstride;
+};
+
+/* one wave: 16 output points x (16 NB) skies over one range of groups; a
+ * workgroup is four waves (64 consecutive points of one output tile) that
+ * share the group records, staged through LDS 256 groups at a time */
+template <int NB>
+__global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
+  __shared__ double2 grec[256]; /* (ugu, vgu) of one block of groups */
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t pbw = vb % a.pbw;
+  const int64_t rest = vb / a.pbw;
+  const int64_t chunk = rest % a.nchunk;
+  const int64_t sg = rest / a.nchunk;
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t pb = pbw * 4 + wave;
+  const bool wave_on = pb < a.pblk;
+  const int64_t pbase = pb * 16; /* pass-relative */
+  const int64_t tile = pbase / a.TP;
+  const int64_t p = pbase + r;
+  const bool okp = wave_on && p < a.npts;
+  const int64_t pp = a.pt0 + (okp ? p : 0);
+  const vw_segparams sp = a.info->sp;
+  const double pl = a.l[pp], pm = a.m[pp], pn = a.n[pp];
+  const double lp = fma(sp.fa, pn, pl), mp = fma(sp.fb, pn, pm);
+  const int64_t nu = a.info->nuniq, q0 = a.info->first_nz;
+  /* group range of this chunk, aligned to 256 groups */
+  const int64_t base = q0 & ~(int64_t)255;
+  const int64_t span = ((nu - base) + a.nchunk - 1) / a.nchunk;
+  const int64_t spanA = (span + 255) & ~(int64_t)255;
+  const int64_t qa = base + chunk * spanA;
+  const int64_t qe = qa + spanA < nu ? qa + spanA : nu;
+
+  vw_f32x4b acc[NB];
+  double sum[NB][4];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    acc[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sum[nb][i] = 0.0;
+  }
+  uint32_t cnt = 0;
+
+  for (int64_t q256 = qa; q256 < qe; q256 += 256) {
+    __syncthreads();
+    {
+      const int64_t q = q256 + threadIdx.x;
+      const GRec gr = a.rec[q < a.nq_pad ? q : 0]; /* records exist up to nq_pad */
+      grec[threadIdx.x] = make_double2(gr.ugu, gr.vgu);
+    }
+    __syncthreads();
+    if (!wave_on) continue;
+    const int64_t kend = qe - q256 < 256 ? qe - q256 : 256;
+    const int nks = (int)((kend + 31) >> 5);
+    const int64_t kb0 = q256 >> 5;
+    auto load_b = [&](int ks, vw_bf16x8 (&b)[4][NB]) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int64_t off = ((kb0 + ks) * a.nsky_pad + (sg * NB + nb) * 16 + r) * 4 + g;
+        b[0][nb] = vw_ld_bf16x8(a.vrh + off);
+        b[1][nb] = vw_ld_bf16x8(a.vrl + off);
+        b[2][nb] = vw_ld_bf16x8(a.vih + off);
+        b[3][nb] = vw_ld_bf16x8(a.vil + off);
+      }
+    };
+    auto step = [&](int ks, const vw_bf16x8 (&b)[4][NB]) {
+      const int64_t qk = q256 + ks * 32 + g * 8; /* first of the lane's 8 groups */
+      uint64_t cw = 0x0101010101010101ull;        /* every group visible */
+      if (a.rep) cw = *(const uint64_t*)(a.gcls + tile * a.nq_pad + qk);
+      float cv[8], sv[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int64_t q = qk + i;
+        const int cls = (int)((cw >> (8 * i)) & 0xffu);
+        bool on = q >= q0 && q < qe && cls != VW_CLS_HIDDEN;
+        if (on && cls == VW_CLS_MIXED) {
+          const double* d = a.gant + q * 10;
+          on = vw_visible(pl, pm, pn, d[0], d[1], d[2], d[3], d[4]) &&
+               vw_visible(pl, pm, pn, d[5], d[6], d[7], d[8], d[9]);
+        }
+        const double2 u = grec[ks * 32 + g * 8 + i];
+        const double x = vw_recon_phase(u.x, u.y, lp, mp);
+        const float f = (float)__builtin_amdgcn_fract(x);
+        const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+        cv[i] = on ? cs : 0.0f;
+        sv[i] = on ? -sn : 0.0f;
+        cnt += on ? 1u : 0u;
+      }
+      vw_bf16x8 ach, acl, ash, asl;
+      vw_split8(cv, ach, acl);
+      vw_split8(sv, ash, asl);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        acc[nb] = vw_mac3(ach, acl, b[0][nb], b[1][nb], acc[nb]);
+        acc[nb] = vw_mac3(ash, asl, b[2][nb], b[3][nb], acc[nb]);
+      }
+    };
+#pragma unroll 1
+    for (int ks = 0; ks < nks; ++ks) {
+      vw_bf16x8 b[4][NB];
+      load_b(ks, b);
+      step(ks, b);
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sum[nb][i] += (double)acc[nb][i];
+      acc[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+  if (!wave_on) return;
+  /* the four lanes of a point (r, r + 16, r + 32, r + 48) hold its counts */
+  cnt += (uint32_t)__shfl_xor((int)cnt, 16, 64);
+  cnt += (uint32_t)__shfl_xor((int)cnt, 32, 64);
+  if (sg == 0 && g == 0 && p < a.npts) a.wpart[chunk * a.pstride + p] = cnt;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int64_t sky = (sg * NB + nb) * 16 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t pt = pbase + 4 * g + i;
+      if (pt < a.npts) a.part[(chunk * a.nsky_pad + sky) * a.pstride + pt] = sum[nb][i];
+    }
+  }
+}
+
+/* group-range partials -> accumulators [nsky][npts_all] (add_inplace); the
+ * sky-0 threads also fold the weights */
+__global__ void k_reconb_fold(const double* __restrict__ part, const uint32_t* __restrict__ wpart, int64_t nchunk,
+                              int64_t nsky, int64_t nsky_pad, int64_t pstride, int64_t pt0, int64_t npts,
+                              int64_t npts_all, int weights, double* __restrict__ acc, uint32_t* __restrict__ wacc) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsky * npts) return;
+  const int64_t s = k / npts, p = k - s * npts;
+  double v = part[s * pstride + p];
+  for (int64_t c = 1; c < nchunk; ++c) v = v + part[(c * nsky_pad + s) * pstride + p];
+  acc[s * npts_all + pt0 + p] += v;
+  if (weights && s == 0) {
+    uint32_t w = wpart[p];
+    for (int64_t c = 1; c < nchunk; ++c) w += wpart[c * pstride + p];
+    wacc[pt0 + p] += w;
+  }
+}
+
+/* final maps of every sky: acc / W (blockage 1) or acc / W_total (blockage 0) */
+__global__ void k_normalize_b(const double* __restrict__ acc, const uint32_t* __restrict__ w, int64_t n, int64_t nsky,
+                              int blockage, const DevInfo* __restrict__ info, const int64_t* __restrict__ scatter,
+                              int64_t out_stride, double* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n * nsky) return;
+  const int64_t s = k / n, i = k - s * n;
+  double v;
+  if (blockage) v = w[i] > 0 ? acc[k] / (double)w[i] : 0.0;
+  else {
+    const int64_t wt = info->wtotal;
+    v = acc[k] * (wt > 0 ? 1.0 / (double)wt : 0.0);
+  }
+  out[s * out_stride + (scatter ? scatter[i] : i)] = v;
+}
+
+/* zero fill (kept out of hipMemsetAsync so the batched step can be captured
+ * in a graph) */
+__global__ void k_fill0(double* __restrict__ p, int64_t n) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+#endif

@@ -59,6 +59,7 @@ def default_config(geometry=A.VW_GEOM_CIRCULAR, **kw):
     c.tile_pix = 256
     c.precision = A.VW_PREC_PLAIN
     c.healpix_out = 1
+    c.nsky = 1
     c.gm_km3_s2 = 398600.4418
     if geometry == A.VW_GEOM_LUNAR8:
         c.nside, c.freq_hz, c.n_ant, c.segs = 4096, 1e7, 8, 10
@@ -97,6 +98,11 @@ class Stages:
         self.plan = A.vw_plan()
         self._check(self._f("get_plan")(self._ctx, C.byref(self.plan)))
         self.nh = 0
+        # batched contexts stack per-sky outputs along a leading axis
+        self.nsky = max(1, int(getattr(cfg, "nsky", 1)))
+
+    def _stack(self, n, dtype=np.float64):
+        return np.zeros((self.nsky, n) if self.nsky > 1 else n, dtype)
 
     def _f(self, name):
         return getattr(self._lib, self._p + name)
@@ -132,8 +138,10 @@ class Stages:
 
     # ---- stages ----
     def set_sky(self, B):
+        """B: [npix] for one sky, [nsky][npix] for a batched context."""
         B = np.ascontiguousarray(B, dtype=np.float64)
-        self._check(self._f("set_sky")(self._ctx, _ptr(B), B.size, 1))
+        nsky = B.shape[0] if B.ndim == 2 else 1
+        self._check(self._f("set_sky")(self._ctx, _ptr(B), B.shape[-1], nsky))
 
     def gen_segment(self, t0, tlen, geometry=False):
         self.nh = int(tlen) * self.plan.n_pairs
@@ -147,8 +155,8 @@ class Stages:
         return out
 
     def conj_fill(self):
-        vh = np.zeros(self.nh, np.complex128)
-        vf = np.zeros(2 * self.nh, np.complex128)
+        vh = self._stack(self.nh, np.complex128)
+        vf = self._stack(2 * self.nh, np.complex128)
         self._check(self._f("conj_fill")(self._ctx, _ptr(vh.view(np.float64)), _ptr(vf.view(np.float64))))
         return vh, vf
 
@@ -159,10 +167,12 @@ class Stages:
 
     def finalize(self):
         npix = self.plan.npix
-        Cm = np.zeros(npix)
+        Cm = self._stack(npix)
         W = np.zeros(npix, np.uint32)
         nimg = self.cfg.img_n
-        img = np.zeros((nimg, nimg)) if nimg else None
+        img = None
+        if nimg:
+            img = np.zeros((self.nsky, nimg, nimg) if self.nsky > 1 else (nimg, nimg))
         Wi = np.zeros((nimg, nimg), np.uint32) if nimg else None
         self._check(self._f("finalize")(self._ctx, _ptr(Cm), _ptr(W, C.c_uint32), _ptr(img), _ptr(Wi, C.c_uint32)))
         return Cm, W, img, Wi
@@ -236,7 +246,7 @@ class Pipeline(Stages):
         super().__init__(load_library(), "vw_", cfg)
 
     def viss_half(self):
-        out = np.zeros(self.nh, np.complex128)
+        out = self._stack(self.nh, np.complex128)
         self._check(self._lib.vw_viss_half(self._ctx, _ptr(out.view(np.float64))))
         return out
 
@@ -253,16 +263,16 @@ class Pipeline(Stages):
 
     def accumulators(self):
         """Raw per-day accumulators: (C, W, G, GW, wtotal) before normalisation."""
-        C_ = np.empty(self.plan.npix)
+        C_ = self._stack(self.plan.npix)
         W = np.empty(self.plan.npix, np.uint32)
         n = max(self.plan.n_img_pts, 1)
-        G = np.empty(n)
+        G = self._stack(n)
         GW = np.empty(n, np.uint32)
         wt = C.c_int64(0)
         self._check(self._lib.vw_get_accumulators(self._ctx, _ptr(C_, C.c_double), _ptr(W, C.c_uint32),
                                                   _ptr(G, C.c_double), _ptr(GW, C.c_uint32), C.byref(wt)))
         k = self.plan.n_img_pts
-        return C_, W, G[:k], GW[:k], wt.value
+        return C_, W, G[..., :k], GW[:k], wt.value
 
     def set_accumulators(self, C_, W, G, GW, wtotal):
         keep, ptrs = [], []
