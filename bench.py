@@ -17,6 +17,14 @@ folded in FP64) is timed on the same workload and its error is stated
 against the plain FP64 result (which matches the CPU FP64 oracle to 1e-9).
 So is the FP64 antenna-factorized Viss option (viss_form 2, "plain_ant"),
 whose phases differ from the per-row ones by FP64 rounding.
+
+The batched leg ("batched") runs 64 sky realisations (log-normal(0,1)
+diffuse field plus Poisson(100) point sources, capped at 200, with
+Pareto(1.5) flux) through the same geometry on one uneven segment schedule
+(lengths drawn uniformly from [16, 256] epochs): Viss and recon as BF16
+hi/lo matrix-core products of on-the-fly phasor tiles against the sky /
+group-visibility matrix.  Its rate counts every sky's terms; its error is
+stated against the plain FP64 path for four of the skies.
 """
 import argparse
 import json
@@ -183,6 +191,97 @@ def _cpu_baseline(sky, threads, vraw0, full_segment):
     return dict(value=terms / dt / 1e9, unit="Gterms/s", cores=threads, kind="port", sample=sample)
 
 
+BATCH_NSKY = 64
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def _uneven_schedule(n_epochs, seed=4242, lo=16, hi=256):
+    """segment lengths drawn uniformly from [lo, hi] epochs (numpy's seeded
+    PCG64 generator) until the run's epochs are covered; the last segment
+    takes the remainder"""
+    rng = np.random.default_rng(seed)
+    out, tot = [], 0
+    while tot < n_epochs:
+        n = int(rng.integers(lo, hi + 1))
+        n = min(n, n_epochs - tot)
+        out.append(n)
+        tot += n
+    return out
+
+
+def _batched_leg(steps, warmup, workload):
+    from viss_optimize_ws_amd import Pipeline, default_config, make_sky
+
+    W = dict(WORKLOAD)
+    W.pop("segs", None)
+    sched = _uneven_schedule(W["n_epochs"])
+    npix = 12 * W["nside"] ** 2
+    skies = np.stack([make_sky(7000 + s, npix, 1) for s in range(BATCH_NSKY)])
+    cfg_b = default_config(1, precision=1, nsky=BATCH_NSKY, seg_tlen=sched, serial_stages=int(SERIAL), **W)
+    cfg_p = default_config(1, precision=0, seg_tlen=sched, **W)
+    # plain FP64 reference for four skies, and the per-segment key counts
+    ref, nkeys = [], []
+    with Pipeline(cfg_p) as p:
+        nimg = p.plan.n_img_pts
+        for s in range(4):
+            p.set_sky(skies[s])
+            p.reset_accumulators()
+            for t0, tl in p.schedule():
+                p.gen_segment(t0, tl)
+                p.viss_half()
+                p.conj_fill()
+                info = p.recon_segment()
+                if s == 0:
+                    nkeys.append(info["nkeys"])
+            ref.append(p.finalize())
+    terms_sky = sum(npix * tl * (W["n_ant"] * (W["n_ant"] - 1) // 2) for tl in sched) + (npix + nimg) * sum(nkeys)
+    with Pipeline(cfg_b) as b:
+        b.set_sky(skies)
+        b.reset_accumulators()
+        b.run_all()
+        Cb, Wb, Ib, _ = b.finalize()
+        sec, tim = _time_steps(b, steps, warmup)
+    if not SERIAL:
+        with Pipeline(default_config(1, precision=1, nsky=BATCH_NSKY, seg_tlen=sched, serial_stages=1, **W)) as b2:
+            b2.set_sky(skies)
+            _, tim = _time_steps(b2, steps, warmup)
+    err = {}
+    for name, k in (("C_healpix", 0), ("lm_image", 2)):
+        got = Cb if k == 0 else Ib
+        mx = [float(np.max(np.abs(got[s] - ref[s][k])) / np.max(np.abs(ref[s][k]))) for s in range(4)]
+        rm = [float(np.sqrt(np.mean((got[s] - ref[s][k]) ** 2)) / np.sqrt(np.mean(ref[s][k] ** 2))) for s in range(4)]
+        err[name] = {"max_rel": max(mx), "rms_rel": max(rm)}
+    weights_equal = bool(all(np.array_equal(Wb, r[1]) for r in ref))
+    nh_tot = sum(tl * (W["n_ant"] * (W["n_ant"] - 1) // 2) for tl in sched)
+    viss_ms = tim["viss_kernel_ms"] / steps
+    recon_ms = tim["recon_kernel_ms"] / steps
+    # matrix-core work as formulated: every (row, pixel) and (point, group)
+    # term of every sky takes 2 real multiply-adds, each as 3 BF16 products
+    viss_flops = 12.0 * nh_tot * npix * BATCH_NSKY
+    recon_flops = 12.0 * (npix + nimg) * sum(nkeys) * BATCH_NSKY
+    return {
+        "value": round(terms_sky * BATCH_NSKY / sec / 1e9, 2),
+        "unit": "Gterms/s",
+        "ms_per_step": round(sec * 1e3, 4),
+        "nsky": BATCH_NSKY,
+        "terms_per_step": terms_sky * BATCH_NSKY,
+        "schedule_epochs": sched,
+        "dtype": "f64 phase, f32 hardware sincos, bf16 hi/lo matrix-core products with f32 accumulation, f64 sums",
+        "data": "synthetic: 64 skies log-normal(0,1) + Poisson(100) (cap 200) Pareto(1.5) point sources, seeds 7000-7063",
+        "err_vs_fp64_4_skies": err,
+        "weights_bitexact": weights_equal,
+        "roofline_viss": {"kernel": "k_vissb", "bound": "mfma", "avg_launch_ms": round(viss_ms / len(sched), 4),
+                          "achieved": round(viss_flops / (viss_ms * 1e-3) / 1e12, 2), "peak": BF16_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(viss_flops / (viss_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                          "flops": "12 per (half row, pixel, sky), culled terms included"},
+        "roofline_recon": {"kernel": "k_reconb", "bound": "mfma", "stage_kernel_ms_per_step": round(recon_ms, 4),
+                           "achieved": round(recon_flops / (recon_ms * 1e-3) / 1e12, 2), "peak": BF16_PEAK_TFLOPS,
+                           "unit": "TFLOP/s",
+                           "frac": round(recon_flops / (recon_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                           "flops": "12 per (output point, group, sky)"},
+    }
+
+
 def _copy_calibration(mib):
     """A plain copy kernel of known size: n bytes read and n bytes written per
     launch.  Its FETCH_SIZE / WRITE_SIZE readings give the factor that turns
@@ -207,6 +306,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-ant", action="store_true", help="skip the FP64 antenna-factorized Viss leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-skies leg")
     ap.add_argument("--serial", action="store_true",
                     help="run every stage on one stream (no overlap of a segment's recon with the next Viss)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nside128",
@@ -338,6 +438,9 @@ def main():
                             "ms_per_step": round(sec_a * 1e3, 4), "dtype": "f64",
                             "viss_kernel_avg_ms": round(tim_a_iso["viss_kernel_ms"] / max(tim_a_iso["n_viss_launch"], 1), 4),
                             "viss_max_rel_err_vs_per_row": float(np.max(np.abs(vr_a - vr_p)) / np.max(np.abs(vr_p)))}
+    if not args.no_batch:
+        out["batched"] = _batched_leg(args.steps, args.warmup, args.workload)
+        out["batched"]["speedup_vs_plain"] = round(out["batched"]["value"] / value, 3)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = _cpu_baseline(sky, threads, per_seg[0]["vraw"], args.workload == "nside128")

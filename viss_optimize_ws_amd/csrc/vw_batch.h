@@ -37,25 +37,28 @@
 typedef __bf16 vw_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float vw_f32x4b __attribute__((ext_vector_type(4)));
 
-/* x = hi + lo: hi keeps the upper 16 bits of x (exact BF16), lo = bf16(x - hi)
- * rounded to nearest even; x - hi is exact in FP32 and |x - hi| < 2^-7 |x|,
- * so hi + lo is within 2^-16 |x|.  Per pair of values: two masks, two
- * subtractions, one byte permute for the hi pair, one conversion for the lo
- * pair. */
+typedef float vw_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 vw_bf16x2 __attribute__((ext_vector_type(2)));
+
+/* x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both rounded to
+ * nearest even: x - hi is exact in FP32 and |x - hi| <= 2^-9 |x|, so
+ * |x - hi - lo| <= 2^-18 |x| and the omitted lo*lo product of a split
+ * multiply is below 2^-18 of the term.  Per pair of values: one packed
+ * conversion, a shift and a mask back to FP32, two subtractions, one packed
+ * conversion of the remainders. */
 __device__ __forceinline__ void vw_split8(const float (&x)[8], vw_bf16x8& hi, vw_bf16x8& lo) {
-  uint32_t h[4];
-  float l[8];
+  uint32_t h[4], l[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const uint32_t a = __builtin_bit_cast(uint32_t, x[2 * j]) & 0xffff0000u;
-    const uint32_t b = __builtin_bit_cast(uint32_t, x[2 * j + 1]) & 0xffff0000u;
-    l[2 * j] = x[2 * j] - __builtin_bit_cast(float, a);
-    l[2 * j + 1] = x[2 * j + 1] - __builtin_bit_cast(float, b);
-    h[j] = __builtin_amdgcn_perm(b, a, 0x07060302u); /* (b.hi << 16) | a.hi */
+    const vw_f32x2 v = {x[2 * j], x[2 * j + 1]};
+    const uint32_t hp = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, vw_bf16x2));
+    const float h0 = __builtin_bit_cast(float, hp << 16), h1 = __builtin_bit_cast(float, hp & 0xffff0000u);
+    const vw_f32x2 rr = {x[2 * j] - h0, x[2 * j + 1] - h1};
+    h[j] = hp;
+    l[j] = __builtin_bit_cast(uint32_t, __builtin_convertvector(rr, vw_bf16x2));
   }
   hi = __builtin_bit_cast(vw_bf16x8, h);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) lo[i] = (__bf16)l[i];
+  lo = __builtin_bit_cast(vw_bf16x8, l);
 }
 
 __device__ __forceinline__ vw_bf16x8 vw_ld_bf16x8(const uint4* p) {
@@ -110,12 +113,55 @@ struct VissBArgs {
   double2* part;             /* [nchunk][nsky_pad][nh_pad]                        */
 };
 
-/* one wave: 16 half rows x (16 NB) skies over the tiles of one chunk.  A
- * workgroup is four waves (64 consecutive rows) on the same pixels, which it
- * stages through LDS one 256-pixel sub-tile at a time. */
+/* LDS staging of one k-step of a packed operand: NARR arrays x (16 NB)
+ * rows of 32 BF16, each row padded to 40 BF16 (80 B) so that the 16 rows a
+ * 16-lane phase reads fall on disjoint banks.  A workgroup loads step k + 1
+ * into registers (16 B per thread and array) while it computes step k, then
+ * stores them into the other buffer. */
+#define VW_LROW 40
+typedef unsigned int vw_u32x4 __attribute__((ext_vector_type(4)));
+template <int NARR, int NB>
+struct VwStage {
+  static constexpr int ROWS = NARR * 16 * NB; /* LDS rows per k-step */
+  vw_u32x4 r0, r1, r2, r3;
+  /* thread t < 64 NB holds the 16-byte unit t of every array's block kb:
+   * row row0 + t / 4 of the sky group, quarter t % 4 of its 32 values */
+  __device__ __forceinline__ void load(const uint4* p0, const uint4* p1, const uint4* p2, const uint4* p3, int64_t kb,
+                                       int64_t nsky_pad, int64_t row0) {
+    const int t = (int)threadIdx.x;
+    if (t < 64 * NB) {
+      const int64_t off = (kb * nsky_pad + row0 + (t >> 2)) * 4 + (t & 3);
+      r0 = *(const vw_u32x4*)(p0 + off);
+      if (NARR > 1) r1 = *(const vw_u32x4*)(p1 + off);
+      if (NARR > 2) r2 = *(const vw_u32x4*)(p2 + off);
+      if (NARR > 3) r3 = *(const vw_u32x4*)(p3 + off);
+    }
+  }
+  __device__ __forceinline__ void store(__bf16* buf) const {
+    const int t = (int)threadIdx.x;
+    if (t < 64 * NB) {
+      __bf16* d = buf + (t >> 2) * VW_LROW + (t & 3) * 8;
+      *(vw_u32x4*)d = r0;
+      if (NARR > 1) *(vw_u32x4*)(d + 16 * NB * VW_LROW) = r1;
+      if (NARR > 2) *(vw_u32x4*)(d + 32 * NB * VW_LROW) = r2;
+      if (NARR > 3) *(vw_u32x4*)(d + 48 * NB * VW_LROW) = r3;
+    }
+  }
+};
+
+__device__ __forceinline__ vw_bf16x8 vw_lds_bf16x8(const __bf16* buf, int row, int g) {
+  return *(const vw_bf16x8*)(buf + row * VW_LROW + g * 8);
+}
+
+/* one wave: 16 half rows x (16 NB) skies over the tiles of one chunk; a
+ * workgroup is four waves (64 consecutive rows) on the same pixels.  Per
+ * k-step of 32 pixels the workgroup stages the sky operand (and, per
+ * 256-pixel sub-tile, l, m, n - 1, n) through double-buffered LDS. */
 template <int NB>
 __global__ __launch_bounds__(256) void k_vissb(VissBArgs a) {
-  __shared__ double px[4][256]; /* l, m, n - 1, n of one sub-tile */
+  using Stage = VwStage<2, NB>;
+  __shared__ __attribute__((aligned(16))) __bf16 bsh[2][Stage::ROWS * VW_LROW];
+  __shared__ double px[2][4][256]; /* l, m, n - 1, n of a sub-tile */
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t rbw = vb % a.nrbw;
   const int64_t rest = vb / a.nrbw;
@@ -132,6 +178,27 @@ __global__ __launch_bounds__(256) void k_vissb(VissBArgs a) {
   const int64_t tend = a.tile0 + a.ntile;
   const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
   const int64_t nsub = a.TP / 256;
+  const int64_t st0 = t0 * nsub;
+  const int64_t K = (t1 - t0) * nsub * 8; /* k-steps of 32 pixels */
+  const int64_t row0 = sg * 16 * NB;
+
+  Stage stg;
+  double pr0 = 0.0, pr1 = 0.0, pr2 = 0.0, pr3 = 0.0; /* this thread's pixel of the next sub-tile */
+#define VW_P_LOAD(st_)                                 \
+  do {                                                 \
+    const int64_t p_ = (st_) * 256 + threadIdx.x;      \
+    const bool ok_ = p_ < a.npix;                      \
+    const int64_t q_ = ok_ ? p_ : 0;                   \
+    pr0 = ok_ ? a.l[q_] : 0.0;                         \
+    pr1 = ok_ ? a.m[q_] : 0.0;                         \
+    pr2 = ok_ ? a.nm1[q_] : 0.0;                       \
+    pr3 = ok_ ? a.n[q_] : 0.0;                         \
+  } while (0)
+#define VW_P_STORE(b_)                                                        \
+  do {                                                                        \
+    px[b_][0][threadIdx.x] = pr0; px[b_][1][threadIdx.x] = pr1;               \
+    px[b_][2][threadIdx.x] = pr2; px[b_][3][threadIdx.x] = pr3;               \
+  } while (0)
 
   vw_f32x4b accR[NB], accI[NB];
   double sumR[NB][4], sumI[NB][4];
@@ -143,53 +210,53 @@ __global__ __launch_bounds__(256) void k_vissb(VissBArgs a) {
     for (int i = 0; i < 4; ++i) { sumR[nb][i] = 0.0; sumI[nb][i] = 0.0; }
   }
 
-  for (int64_t st = t0 * nsub; st < t1 * nsub; ++st) {
-    const int64_t tile = st / nsub;
-    const int64_t pbase = st * 256;
-    __syncthreads(); /* the previous sub-tile is consumed */
-    {
-      const int64_t p = pbase + threadIdx.x;
-      const bool okp = p < a.npix;
-      const int64_t pp = okp ? p : 0;
-      px[0][threadIdx.x] = okp ? a.l[pp] : 0.0;
-      px[1][threadIdx.x] = okp ? a.m[pp] : 0.0;
-      px[2][threadIdx.x] = okp ? a.nm1[pp] : 0.0;
-      px[3][threadIdx.x] = okp ? a.n[pp] : 0.0;
-    }
-    __syncthreads();
-    if (!wave_on) continue;
-    /* rows past nh carry the hidden class in the table */
-    const int cls = (int)a.cls[(tile - a.tile0) * a.nh_pad + h];
-    if (__ballot(cls != VW_CLS_HIDDEN) == 0) continue;
-    const bool any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
-    double ax = 0, ay = 0, az = 0, ai = 0, ac = 0, bx = 0, by = 0, bz = 0, bi = 0, bc = 0;
-    if (any_mixed && cls == VW_CLS_MIXED) {
-      const double* qa = a.ant8 + (int64_t)rec.ia * 8;
-      const double* qb = a.ant8 + (int64_t)rec.ib * 8;
-      ax = qa[0]; ay = qa[1]; az = qa[2]; ai = qa[3]; ac = qa[7];
-      bx = qb[0]; by = qb[1]; bz = qb[2]; bi = qb[3]; bc = qb[7];
-    }
-    const int64_t kb0 = pbase >> 5;
-    auto load_b = [&](int ks, vw_bf16x8 (&bh)[NB], vw_bf16x8 (&bl)[NB]) {
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int64_t off = ((kb0 + ks) * a.nsky_pad + (sg * NB + nb) * 16 + r) * 4 + g;
-        bh[nb] = vw_ld_bf16x8(a.skyh + off);
-        bl[nb] = vw_ld_bf16x8(a.skyl + off);
+  /* prologue: step 0 and sub-tile 0 */
+  VW_P_LOAD(st0);
+  stg.load(a.skyh, a.skyl, nullptr, nullptr, st0 * 8, a.nsky_pad, row0);
+  VW_P_STORE(0);
+  stg.store(bsh[0]);
+  __syncthreads();
+
+  int cls = VW_CLS_HIDDEN;
+  bool act = false, any_mixed = false;
+  double ax = 0, ay = 0, az = 0, ai = 0, ac = 0, bx = 0, by = 0, bz = 0, bi = 0, bc = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    const int64_t sidx = k >> 3;
+    const int ks = (int)(k & 7);
+    const int pbuf = (int)(sidx & 1), bbuf = (int)(k & 1);
+    const bool more = k + 1 < K;
+    if (more) stg.load(a.skyh, a.skyl, nullptr, nullptr, st0 * 8 + k + 1, a.nsky_pad, row0);
+    if (ks == 0) {
+      /* a new sub-tile: its pixels for the next one, this one's classes */
+      if (sidx + 1 < K / 8) VW_P_LOAD(st0 + sidx + 1);
+      const int64_t tile = (st0 + sidx) / nsub;
+      /* rows past nh carry the hidden class in the table */
+      cls = wave_on ? (int)a.cls[(tile - a.tile0) * a.nh_pad + h] : VW_CLS_HIDDEN;
+      act = __ballot(cls != VW_CLS_HIDDEN) != 0;
+      any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
+      if (any_mixed && cls == VW_CLS_MIXED) {
+        const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+        const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+        ax = qa[0]; ay = qa[1]; az = qa[2]; ai = qa[3]; ac = qa[7];
+        bx = qb[0]; by = qb[1]; bz = qb[2]; bi = qb[3]; bc = qb[7];
       }
-    };
-    auto step = [&](int ks, const vw_bf16x8 (&bh)[NB], const vw_bf16x8 (&bl)[NB]) {
+    }
+    if (act) {
+      const double* pl_ = px[pbuf][0];
+      const double* pm_ = px[pbuf][1];
+      const double* pz_ = px[pbuf][2];
+      const double* pn_ = px[pbuf][3];
       float cv[8], sv[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const int k = ks * 32 + g * 8 + i;
-        const double pl = px[0][k], pm = px[1][k], pz = px[2][k];
+        const int kk = ks * 32 + g * 8 + i;
+        const double pl = pl_[kk], pm = pm_[kk], pz = pz_[kk];
         const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl, pm, pz);
         const float f = (float)__builtin_amdgcn_fract(x);
         const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
         bool on = cls == VW_CLS_VISIBLE;
         if (any_mixed && cls == VW_CLS_MIXED) {
-          const double pn = px[3][k];
+          const double pn = pn_[kk];
           on = vw_visible(pl, pm, pn, ax, ay, az, ai, ac) && vw_visible(pl, pm, pn, bx, by, bz, bi, bc);
         }
         cv[i] = on ? cs : 0.0f;
@@ -198,30 +265,38 @@ __global__ __launch_bounds__(256) void k_vissb(VissBArgs a) {
       vw_bf16x8 ach, acl, ash, asl;
       vw_split8(cv, ach, acl);
       vw_split8(sv, ash, asl);
+      const __bf16* bb = bsh[bbuf];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
-        accR[nb] = vw_mac3(ach, acl, bh[nb], bl[nb], accR[nb]);
-        accI[nb] = vw_mac3(ash, asl, bh[nb], bl[nb], accI[nb]);
+        const vw_bf16x8 bh = vw_lds_bf16x8(bb, nb * 16 + r, g);
+        const vw_bf16x8 bl = vw_lds_bf16x8(bb, 16 * NB + nb * 16 + r, g);
+        accR[nb] = vw_mac3(ach, acl, bh, bl, accR[nb]);
+        accI[nb] = vw_mac3(ash, asl, bh, bl, accI[nb]);
       }
-    };
-#pragma unroll 1
-    for (int ks = 0; ks < 8; ++ks) {
-      vw_bf16x8 bh[NB], bl[NB];
-      load_b(ks, bh, bl);
-      step(ks, bh, bl);
-    }
-    /* FP64 sums across sub-tiles */
+      if (ks == 7) {
+        /* FP64 sums across sub-tiles */
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
+        for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        sumR[nb][i] += (double)accR[nb][i];
-        sumI[nb][i] += (double)accI[nb][i];
+          for (int i = 0; i < 4; ++i) {
+            sumR[nb][i] += (double)accR[nb][i];
+            sumI[nb][i] += (double)accI[nb][i];
+          }
+          accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+          accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+        }
       }
-      accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
-      accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
     }
+    if (more) {
+      /* buffers of step k - 1 / sub-tile sidx - 1: every wave passed the
+       * barrier after reading them */
+      stg.store(bsh[bbuf ^ 1]);
+      if (ks == 7) VW_P_STORE(pbuf ^ 1);
+    }
+    __syncthreads();
   }
+#undef VW_P_LOAD
+#undef VW_P_STORE
   if (!wave_on) return;
   /* D[4 g + i][r]: half row rb * 16 + 4 g + i, sky column r */
 #pragma unroll
@@ -337,11 +412,16 @@ struct ReconBArgs {
 };
 
 /* one wave: 16 output points x (16 NB) skies over one range of groups; a
- * workgroup is four waves (64 consecutive points of one output tile) that
- * share the group records, staged through LDS 256 groups at a time */
+ * workgroup is four waves (64 consecutive points of one output tile).  Per
+ * k-step of 32 groups the workgroup stages the packed group visibilities,
+ * the groups' cell centres and their class bytes for the tile through
+ * double-buffered LDS. */
 template <int NB>
 __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
-  __shared__ double2 grec[256]; /* (ugu, vgu) of one block of groups */
+  using Stage = VwStage<4, NB>;
+  __shared__ __attribute__((aligned(16))) __bf16 bsh[2][Stage::ROWS * VW_LROW];
+  __shared__ double2 grec[2][32];                      /* (ugu, vgu)      */
+  __shared__ __attribute__((aligned(16))) uint8_t gcl[2][32]; /* classes */
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t pbw = vb % a.pbw;
   const int64_t rest = vb / a.pbw;
@@ -351,8 +431,8 @@ __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
   const int r = lane & 15, g = lane >> 4;
   const int64_t pb = pbw * 4 + wave;
   const bool wave_on = pb < a.pblk;
-  const int64_t pbase = pb * 16; /* pass-relative */
-  const int64_t tile = pbase / a.TP;
+  const int64_t pbase = pb * 16;          /* pass-relative */
+  const int64_t tile = (pbw * 64) / a.TP; /* the workgroup's 64 points lie in one tile */
   const int64_t p = pbase + r;
   const bool okp = wave_on && p < a.npts;
   const int64_t pp = a.pt0 + (okp ? p : 0);
@@ -366,6 +446,33 @@ __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
   const int64_t spanA = (span + 255) & ~(int64_t)255;
   const int64_t qa = base + chunk * spanA;
   const int64_t qe = qa + spanA < nu ? qa + spanA : nu;
+  const int64_t K = qe > qa ? (qe - qa + 31) >> 5 : 0; /* uniform over the workgroup */
+  const int64_t row0 = sg * 16 * NB;
+
+  Stage stg;
+  /* threads 0-31: a cell centre (ugu, vgu); 32-33: 16 class bytes */
+  double xa = 0.0, xb = 0.0;
+  vw_u32x4 xc = {0u, 0u, 0u, 0u};
+#define VW_X_LOAD(k_)                                                                       \
+  do {                                                                                      \
+    const int t_ = (int)threadIdx.x;                                                        \
+    const int64_t qk_ = qa + 32 * (k_);                                                     \
+    if (t_ < 32) {                                                                          \
+      const int64_t q_ = qk_ + t_;                                                          \
+      const double* gr_ = (const double*)(a.rec + (q_ < a.nq_pad ? q_ : 0)); /* up to nq_pad */ \
+      xa = gr_[0];                                                                          \
+      xb = gr_[1];                                                                          \
+    } else if (t_ < 34 && a.rep) {                                                          \
+      /* qk + 32 <= nq_pad: qk < qe <= nq_pad and both are multiples of 32 */               \
+      xc = *(const vw_u32x4*)(a.gcls + tile * a.nq_pad + qk_ + 16 * (t_ - 32));             \
+    }                                                                                       \
+  } while (0)
+#define VW_X_STORE(b_)                                                   \
+  do {                                                                   \
+    const int t_ = (int)threadIdx.x;                                     \
+    if (t_ < 32) grec[b_][t_] = make_double2(xa, xb);                    \
+    else if (t_ < 34 && a.rep) *(vw_u32x4*)(&gcl[b_][16 * (t_ - 32)]) = xc; \
+  } while (0)
 
   vw_f32x4b acc[NB];
   double sum[NB][4];
@@ -377,32 +484,24 @@ __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
   }
   uint32_t cnt = 0;
 
-  for (int64_t q256 = qa; q256 < qe; q256 += 256) {
-    __syncthreads();
-    {
-      const int64_t q = q256 + threadIdx.x;
-      const GRec gr = a.rec[q < a.nq_pad ? q : 0]; /* records exist up to nq_pad */
-      grec[threadIdx.x] = make_double2(gr.ugu, gr.vgu);
+  if (K > 0) {
+    stg.load(a.vrh, a.vrl, a.vih, a.vil, qa >> 5, a.nsky_pad, row0);
+    VW_X_LOAD(0);
+    stg.store(bsh[0]);
+    VW_X_STORE(0);
+  }
+  __syncthreads();
+  for (int64_t k = 0; k < K; ++k) {
+    const int buf = (int)(k & 1);
+    const bool more = k + 1 < K;
+    if (more) {
+      stg.load(a.vrh, a.vrl, a.vih, a.vil, (qa >> 5) + k + 1, a.nsky_pad, row0);
+      VW_X_LOAD(k + 1);
     }
-    __syncthreads();
-    if (!wave_on) continue;
-    const int64_t kend = qe - q256 < 256 ? qe - q256 : 256;
-    const int nks = (int)((kend + 31) >> 5);
-    const int64_t kb0 = q256 >> 5;
-    auto load_b = [&](int ks, vw_bf16x8 (&b)[4][NB]) {
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int64_t off = ((kb0 + ks) * a.nsky_pad + (sg * NB + nb) * 16 + r) * 4 + g;
-        b[0][nb] = vw_ld_bf16x8(a.vrh + off);
-        b[1][nb] = vw_ld_bf16x8(a.vrl + off);
-        b[2][nb] = vw_ld_bf16x8(a.vih + off);
-        b[3][nb] = vw_ld_bf16x8(a.vil + off);
-      }
-    };
-    auto step = [&](int ks, const vw_bf16x8 (&b)[4][NB]) {
-      const int64_t qk = q256 + ks * 32 + g * 8; /* first of the lane's 8 groups */
-      uint64_t cw = 0x0101010101010101ull;        /* every group visible */
-      if (a.rep) cw = *(const uint64_t*)(a.gcls + tile * a.nq_pad + qk);
+    if (wave_on) {
+      const int64_t qk = qa + 32 * k + g * 8; /* first of the lane's 8 groups */
+      uint64_t cw = 0x0101010101010101ull;     /* every group visible */
+      if (a.rep) cw = *(const uint64_t*)(&gcl[buf][g * 8]);
       float cv[8], sv[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -414,7 +513,7 @@ __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
           on = vw_visible(pl, pm, pn, d[0], d[1], d[2], d[3], d[4]) &&
                vw_visible(pl, pm, pn, d[5], d[6], d[7], d[8], d[9]);
         }
-        const double2 u = grec[ks * 32 + g * 8 + i];
+        const double2 u = grec[buf][g * 8 + i];
         const double x = vw_recon_phase(u.x, u.y, lp, mp);
         const float f = (float)__builtin_amdgcn_fract(x);
         const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
@@ -425,25 +524,32 @@ __global__ __launch_bounds__(256) void k_reconb(ReconBArgs a) {
       vw_bf16x8 ach, acl, ash, asl;
       vw_split8(cv, ach, acl);
       vw_split8(sv, ash, asl);
+      const __bf16* bb = bsh[buf];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
-        acc[nb] = vw_mac3(ach, acl, b[0][nb], b[1][nb], acc[nb]);
-        acc[nb] = vw_mac3(ash, asl, b[2][nb], b[3][nb], acc[nb]);
+        acc[nb] = vw_mac3(ach, acl, vw_lds_bf16x8(bb, nb * 16 + r, g),
+                          vw_lds_bf16x8(bb, 16 * NB + nb * 16 + r, g), acc[nb]);
+        acc[nb] = vw_mac3(ash, asl, vw_lds_bf16x8(bb, 32 * NB + nb * 16 + r, g),
+                          vw_lds_bf16x8(bb, 48 * NB + nb * 16 + r, g), acc[nb]);
       }
-    };
-#pragma unroll 1
-    for (int ks = 0; ks < nks; ++ks) {
-      vw_bf16x8 b[4][NB];
-      load_b(ks, b);
-      step(ks, b);
-    }
+      if ((k & 7) == 7 || !more) {
+        /* FP64 sums across blocks of 256 groups */
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
+        for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) sum[nb][i] += (double)acc[nb][i];
-      acc[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+          for (int i = 0; i < 4; ++i) sum[nb][i] += (double)acc[nb][i];
+          acc[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+        }
+      }
     }
+    if (more) {
+      stg.store(bsh[buf ^ 1]);
+      VW_X_STORE(buf ^ 1);
+    }
+    __syncthreads();
   }
+#undef VW_X_LOAD
+#undef VW_X_STORE
   if (!wave_on) return;
   /* the four lanes of a point (r, r + 16, r + 32, r + 48) hold its counts */
   cnt += (uint32_t)__shfl_xor((int)cnt, 16, 64);
