@@ -130,6 +130,13 @@ struct Ctx {
   double *d_brpart = nullptr, *d_brpart2 = nullptr;
   uint32_t *d_bwpart = nullptr, *d_bwpart2 = nullptr;
   int64_t brpart_cap = 0, bwpart_cap = 0;
+  /* work orders of the batched kernels: keys, indices, sorted keys, order
+   * (rows on the Viss stream, groups on the grouping stream) and their
+   * radix-sort scratch */
+  int32_t *d_okey[2] = {}, *d_oidx[2] = {}, *d_okey2[2] = {}, *d_order[2] = {};
+  void* d_otmp[2] = {};
+  size_t otmp_bytes[2] = {};
+  double2* d_orec = nullptr; /* (ugu, vgu) of the group at each work-order position */
   double *d_Cacc = nullptr, *d_Gacc = nullptr;
   uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
   double *d_out = nullptr, *d_iout = nullptr;
@@ -572,6 +579,19 @@ int create_batch(Ctx* c) {
   }
   c->brpart_cap = rp;
   c->bwpart_cap = wp;
+  for (int w = 0; w < 2; ++w) {
+    const int64_t n = w == 0 ? c->nh_pad : c->nq_pad;
+    if ((rc = dalloc(c, &c->d_okey[w], (size_t)n)) || (rc = dalloc(c, &c->d_oidx[w], (size_t)n)) ||
+        (rc = dalloc(c, &c->d_okey2[w], (size_t)n)) || (rc = dalloc(c, &c->d_order[w], (size_t)n)))
+      return rc;
+    HC(rocprim::radix_sort_pairs(nullptr, c->otmp_bytes[w], (const int32_t*)nullptr, (int32_t*)nullptr,
+                                 (const int32_t*)nullptr, (int32_t*)nullptr, (unsigned)n, 0u,
+                                 (unsigned)VW_ORDER_BITS, c->st));
+    uint8_t* tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, c->otmp_bytes[w]))) return rc;
+    c->d_otmp[w] = tmp;
+  }
+  if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad))) return rc;
   if ((rc = dalloc(c, &c->d_brpart, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart, (size_t)wp))) return rc;
   if (c->ngp > 0 && ((rc = dalloc(c, &c->d_brpart2, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart2, (size_t)wp))))
     return rc;
@@ -764,7 +784,7 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
   KC();
   if (groups)
     k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib, c->d_info,
-                                                            c->nq_pad, cls);
+                                                            c->nq_pad, c->batch ? c->d_order[1] : nullptr, cls);
   else
     k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, cls);
   KC();
@@ -819,6 +839,35 @@ int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
   return VW_OK;
 }
 
+/* work order of the batched kernels (which = 0: the segment's half rows on
+ * the Viss stream; 1: its groups on the grouping stream): a stable radix
+ * sort of the antenna-direction pair keys */
+#ifndef VW_GROUP_ORDER
+#define VW_GROUP_ORDER 1
+#endif
+int enq_order(Ctx* c, int which) {
+  hipStream_t st = which == 0 ? c->st : c->stB;
+  const int64_t n = which == 0 ? c->nh_pad : c->nq_pad;
+  if (which == 1 && !VW_GROUP_ORDER) {
+    /* identity order */
+    k_group_keys<<<blocks_for(n), 256, 0, st>>>(c->gs.gia, c->gs.gib, c->d_ant8, c->d_info, n, c->d_okey[1],
+                                                c->d_order[1]);
+    KC();
+    return VW_OK;
+  }
+  if (which == 0)
+    k_row_keys<<<blocks_for(n), 256, 0, st>>>(c->d_rows, c->d_ant8, c->nh, n, c->d_okey[0], c->d_oidx[0]);
+  else
+    k_group_keys<<<blocks_for(n), 256, 0, st>>>(c->gs.gia, c->gs.gib, c->d_ant8, c->d_info, n, c->d_okey[1],
+                                                c->d_oidx[1]);
+  KC();
+  size_t tb = c->otmp_bytes[which];
+  HC(rocprim::radix_sort_pairs(c->d_otmp[which], tb, (const int32_t*)c->d_okey[which], c->d_okey2[which],
+                               (const int32_t*)c->d_oidx[which], c->d_order[which], (unsigned)n, 0u,
+                               (unsigned)VW_ORDER_BITS, st));
+  return VW_OK;
+}
+
 /* one tile pass of the batched Viss: classes, the product kernel, the fold */
 int enq_viss_batch(Ctx* c, int64_t tile0, int64_t nt) {
   int rc;
@@ -830,7 +879,7 @@ int enq_viss_batch(Ctx* c, int64_t tile0, int64_t nt) {
   a.skyh = (const uint4*)c->d_skyh; a.skyl = (const uint4*)c->d_skyl;
   a.npix = c->npix; a.nsky_pad = c->nsky_pad;
   a.tile0 = tile0; a.ntile = nt; a.TP = c->TP;
-  a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = c->d_cls_v;
+  a.rows = c->d_rows; a.ant8 = c->d_ant8; a.cls = c->d_cls_v; a.perm = c->d_order[0];
   a.nh = c->nh; a.nh_pad = c->nh_pad;
   a.tpc = s.tpc; a.nchunk = s.nchunk; a.nrb = s.nrb; a.nrbw = s.nrbw;
   a.part = c->d_bpart;
@@ -864,6 +913,7 @@ int enq_viss(Ctx* c, bool count) {
   for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
     if (c->batch) {
+      if (tile0 == 0 && (rc = enq_order(c, 0))) return rc;
       if ((rc = enq_viss_batch(c, tile0, nt))) return rc;
       continue;
     }
@@ -959,9 +1009,13 @@ int enq_group(Ctx* c) {
                                                   c->gs);
   KC();
   if (c->batch) {
+    int rc;
+    if ((rc = enq_order(c, 1))) return rc;
+    k_group_order_recs<<<blocks_for(c->nq_pad), 256, 0, c->stB>>>(c->d_order[1], c->gs.rec, c->nq_pad, c->d_orec);
+    KC();
     k_group_pack<<<blocks_for(c->nq_pad * c->nsky_pad), 256, 0, c->stB>>>(
         c->d_goff, c->d_idx_sorted, c->d_vh, c->nh, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode, c->d_info,
-        c->nq_pad, c->nsky, c->nsky_pad, c->d_gpk[0], c->d_gpk[1], c->d_gpk[2], c->d_gpk[3]);
+        c->d_order[1], c->nq_pad, c->nsky, c->nsky_pad, c->d_gpk[0], c->d_gpk[1], c->d_gpk[2], c->d_gpk[3]);
     KC();
   }
   ev_end(c, ST_GROUP, c->stB);
@@ -1002,7 +1056,7 @@ int enq_recon_set_batch(Ctx* c, bool img) {
     a.m = img ? c->d_gm : c->d_m;
     a.n = img ? c->d_gn : c->d_n;
     a.pt0 = pt0; a.npts = np; a.TP = c->TP;
-    a.gcls = cls; a.nq_pad = c->nq_pad;
+    a.gcls = cls; a.nq_pad = c->nq_pad; a.gperm = c->d_order[1]; a.orec = c->d_orec;
     a.rec = c->gs.rec; a.gant = c->gs.ant; a.info = c->d_info;
     a.vrh = (const uint4*)c->d_gpk[0]; a.vrl = (const uint4*)c->d_gpk[1];
     a.vih = (const uint4*)c->d_gpk[2]; a.vil = (const uint4*)c->d_gpk[3];

@@ -1100,13 +1100,16 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
   d[5] = qb[0]; d[6] = qb[1]; d[7] = qb[2]; d[8] = qb[3]; d[9] = qb[7];
 }
 
-/* class of (output tile, group) from the representative's antennas */
+/* class of (output tile, group) from the representative's antennas; with an
+ * order (batched recon) entry j of a tile's row holds group order[j] */
 __global__ void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, int64_t nt,
                             const int32_t* __restrict__ gia, const int32_t* __restrict__ gib,
-                            const DevInfo* __restrict__ info, int64_t nq_pad, uint8_t* __restrict__ gcls) {
+                            const DevInfo* __restrict__ info, int64_t nq_pad, const int32_t* __restrict__ order,
+                            uint8_t* __restrict__ gcls) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nt * nq_pad) return;
   int64_t tr = k / nq_pad, q = k - tr * nq_pad;
+  if (order) q = order[q];
   if (q >= info->nuniq) { gcls[k] = VW_CLS_HIDDEN; return; }
   const uint8_t* b = bits + tr * nfr_pad;
   gcls[k] = (uint8_t)vw_pair_class(b[gia[q]], b[gib[q]]);
