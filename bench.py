@@ -237,13 +237,14 @@ def _batched_leg(steps, warmup, workload):
     terms_sky = sum(npix * tl * (W["n_ant"] * (W["n_ant"] - 1) // 2) for tl in sched) + (npix + nimg) * sum(nkeys)
     with Pipeline(cfg_b) as b:
         b.set_sky(skies)
-        b.reset_accumulators()
+        targets = b.autotune(reps=2)  # launch shapes of both batched kernels
         b.run_all()
         Cb, Wb, Ib, _ = b.finalize()
         sec, tim = _time_steps(b, steps, warmup)
     if not SERIAL:
         with Pipeline(default_config(1, precision=1, nsky=BATCH_NSKY, seg_tlen=sched, serial_stages=1, **W)) as b2:
             b2.set_sky(skies)
+            b2.set_batch_targets(*targets)
             _, tim = _time_steps(b2, steps, warmup)
     err = {}
     for name, k in (("C_healpix", 0), ("lm_image", 2)):
@@ -266,6 +267,7 @@ def _batched_leg(steps, warmup, workload):
         "nsky": BATCH_NSKY,
         "terms_per_step": terms_sky * BATCH_NSKY,
         "schedule_epochs": sched,
+        "autotuned_waves_per_launch": {"viss": targets[0], "recon": targets[1]},
         "dtype": "f64 phase, f32 hardware sincos, bf16 hi/lo matrix-core products with f32 accumulation, f64 sums",
         "data": "synthetic: 64 skies log-normal(0,1) + Poisson(100) (cap 200) Pareto(1.5) point sources, seeds 7000-7063",
         "err_vs_fp64_4_skies": err,

@@ -242,6 +242,18 @@ int vw_set_accumulators(void* ctx, const double* C, const uint32_t* W, const dou
 
 int vw_get_timings(void* ctx, vw_timings* out);
 
+/* ---------------- batched launch shapes ---------------- */
+
+/* Waves per launch that the batched Viss and recon kernels size their chunk
+ * counts for (one of 2048, 4096, 8192, 16384, 32768, 65536; defaults 8192
+ * and 16384).  Batched contexts only; no counterpart in the reference. */
+int vw_set_batch_targets(void* ctx, int64_t viss_waves, int64_t recon_waves);
+int vw_get_batch_targets(void* ctx, int64_t* viss_waves, int64_t* recon_waves);
+/* Autotuner: times every target on the schedule's longest segment (reps
+ * timed runs each, HIP events) and keeps the fastest for each kernel; the
+ * accumulators are cleared afterwards.  Needs vw_set_sky. */
+int vw_autotune(void* ctx, int32_t reps, int64_t* viss_waves, int64_t* recon_waves);
+
 /* ---------------- introspection (tests) ---------------- */
 
 /* Resident pixel directions: l, m, n, nm1 (npix each). */

@@ -140,3 +140,37 @@ def test_batch_sky_errors():
             g.set_sky(sk)
         with pytest.raises(VissError):
             g.viss_counts()
+
+
+TARGETS = (2048, 4096, 8192, 16384, 32768, 65536)
+
+
+def test_batch_autotune_and_launch_targets():
+    """the autotuner picks one of the launch targets and clears the
+    accumulators; every launch shape gives the same maps to the mixed bar
+    (only the chunking of the FP64 sums changes) and the same weights"""
+    kw = dict(nside=32, n_ant=16, n_epochs=32, segs=2, img_n=48, precision=1, nsky=32)
+    skies = _skies(32, 12 * 32 * 32, 11)
+    with Pipeline(config(**kw)) as g:
+        g.set_sky(skies)
+        assert g.batch_targets() == (8192, 16384)
+        g.run_all()
+        C0, W0, I0, WI0 = g.finalize()
+        v, r = g.autotune(reps=2)
+        assert v in TARGETS and r in TARGETS and g.batch_targets() == (v, r)
+        g.run_all()  # the autotuner left cleared accumulators
+        C1, W1, I1, WI1 = g.finalize()
+        for t in (2048, 65536):
+            g.set_batch_targets(t, t)
+            g.reset_accumulators()
+            g.run_all()
+            C2, W2, I2, WI2 = g.finalize()
+            assert np.array_equal(W2, W0) and np.array_equal(WI2, WI0)
+            assert rel_err(C2, C0) <= MIXED_TOL and rel_err(I2, I0) <= MIXED_TOL
+        assert np.array_equal(W1, W0) and np.array_equal(WI1, WI0)
+        assert rel_err(C1, C0) <= MIXED_TOL and rel_err(I1, I0) <= MIXED_TOL
+        with pytest.raises(VissError):
+            g.set_batch_targets(1000, 2048)
+    with Pipeline(config(nside=8, n_ant=4, n_epochs=4, segs=1, img_n=0)) as one:
+        with pytest.raises(VissError):
+            one.autotune()

@@ -155,6 +155,9 @@ DEVICE_ONLY = {
     "get_accumulators": (C.c_int, [P, PD, PU32, PD, PU32, PI64]),
     "set_accumulators": (C.c_int, [P, PD, PU32, PD, PU32, i64]),
     "get_viss_counts": (C.c_int, [P, PI64]),
+    "set_batch_targets": (C.c_int, [P, i64, i64]),
+    "get_batch_targets": (C.c_int, [P, PI64, PI64]),
+    "autotune": (C.c_int, [P, i32, PI64, PI64]),
     "make_sky": (C.c_int, [C.c_uint64, i64, i32, PD]),
     "io_load_sky": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, i64, PD, C.c_char_p, i32]),
     "io_write_map": (C.c_int, [C.c_char_p, PD, i64, i32]),

@@ -1531,6 +1531,93 @@ VW_API int vw_set_accumulators(void* p, const double* C, const uint32_t* W, cons
   return sync(c);
 }
 
+/* launch-shape targets of the batched kernels (waves per launch) */
+VW_API int vw_set_batch_targets(void* p, int64_t viss_waves, int64_t recon_waves) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->batch) return c->fail(VW_ERR_ARG, "launch targets apply to batched contexts (nsky > 1)");
+  auto ok = [](int64_t t) {
+    for (int64_t v : VW_BATCH_TARGETS)
+      if (v == t) return true;
+    return false;
+  };
+  if (!ok(viss_waves) || !ok(recon_waves))
+    return c->fail(VW_ERR_ARG, "target must be one of 2048, 4096, 8192, 16384, 32768, 65536");
+  c->vb_target = viss_waves;
+  c->rb_target = recon_waves;
+  return VW_OK;
+}
+
+VW_API int vw_get_batch_targets(void* p, int64_t* viss_waves, int64_t* recon_waves) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (viss_waves) *viss_waves = c->vb_target;
+  if (recon_waves) *recon_waves = c->rb_target;
+  return VW_OK;
+}
+
+/* Autotuner: on the schedule's longest segment, times the batched Viss
+ * kernel and the recon stage for every launch target (reps runs each, HIP
+ * events on the context's streams) and keeps the fastest.  The accumulators
+ * are cleared afterwards. */
+VW_API int vw_autotune(void* p, int32_t reps, int64_t* viss_waves, int64_t* recon_waves) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->batch) return c->fail(VW_ERR_ARG, "the autotuner tunes batched contexts (nsky > 1)");
+  if (!c->have_sky) return c->fail(VW_ERR_STATE, "vw_autotune needs vw_set_sky");
+  if (reps <= 0 || reps > 100) return c->fail(VW_ERR_ARG, "reps must be in [1, 100]");
+  int rc;
+  if ((rc = join_recon(c)) || (rc = sync(c))) return rc;
+  int longest = 0;
+  for (int k = 1; k < c->segs; ++k)
+    if (c->seg_tlen[k] > c->seg_tlen[longest]) longest = k;
+  if ((rc = enq_gen(c, c->seg_t0[longest], c->seg_tlen[longest]))) return rc;
+  hipEvent_t e0, e1;
+  HC(hipEventCreate(&e0));
+  HC(hipEventCreate(&e1));
+  auto timed = [&](hipStream_t st, auto&& body, float* ms) -> int {
+    int r = body(); /* warm-up */
+    if (r) return r;
+    HC(hipEventRecord(e0, st));
+    for (int i = 0; i < reps && !r; ++i) r = body();
+    if (r) return r;
+    HC(hipEventRecord(e1, st));
+    HC(hipEventSynchronize(e1));
+    HC(hipEventElapsedTime(ms, e0, e1));
+    return VW_OK;
+  };
+  float best = 1e30f, ms = 0.f;
+  int64_t bv = c->vb_target, br = c->rb_target;
+  for (int64_t t : VW_BATCH_TARGETS) {
+    c->vb_target = t;
+    if ((rc = timed(c->st, [&]() { return enq_viss(c, false); }, &ms))) break;
+    if (ms < best) { best = ms; bv = t; }
+  }
+  c->vb_target = bv;
+  if (!rc) rc = enq_viss(c, false);
+  if (!rc) rc = enq_conj(c);
+  if (!rc) {
+    best = 1e30f;
+    for (int64_t t : VW_BATCH_TARGETS) {
+      c->rb_target = t;
+      /* the recon stage ends on stB after joining the image set's stream */
+      HC(hipStreamWaitEvent(c->stB, c->ev_conj, 0));
+      if ((rc = timed(c->stB, [&]() { return enq_recon(c); }, &ms))) break;
+      if (ms < best) { best = ms; br = t; }
+    }
+    c->rb_target = br;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if ((rc = vw_reset_accumulators(c))) return rc;
+  c->have_seg = c->have_viss = c->have_conj = c->have_groups = false;
+  c->tim = vw_timings{};
+  if (viss_waves) *viss_waves = c->vb_target;
+  if (recon_waves) *recon_waves = c->rb_target;
+  return VW_OK;
+}
+
 VW_API int vw_get_timings(void* p, vw_timings* o) {
   Ctx* c = as_ctx(p);
   if (!c || !o) return VW_ERR_ARG;

@@ -285,6 +285,21 @@ class Pipeline(Stages):
                 ptrs.append(_ptr(keep[-1], ct))
         self._check(self._lib.vw_set_accumulators(self._ctx, *ptrs, int(wtotal)))
 
+    def set_batch_targets(self, viss_waves, recon_waves):
+        self._check(self._lib.vw_set_batch_targets(self._ctx, int(viss_waves), int(recon_waves)))
+
+    def batch_targets(self):
+        v, r = C.c_int64(0), C.c_int64(0)
+        self._check(self._lib.vw_get_batch_targets(self._ctx, C.byref(v), C.byref(r)))
+        return v.value, r.value
+
+    def autotune(self, reps=3):
+        """Times every launch target of the batched kernels and keeps the
+        fastest; clears the accumulators.  Returns (viss, recon) targets."""
+        v, r = C.c_int64(0), C.c_int64(0)
+        self._check(self._lib.vw_autotune(self._ctx, int(reps), C.byref(v), C.byref(r)))
+        return v.value, r.value
+
     def timings(self):
         t = A.vw_timings()
         self._check(self._lib.vw_get_timings(self._ctx, C.byref(t)))
