@@ -23,7 +23,7 @@ CLI = os.path.join(ROOT, "viss_optimize_ws_amd", "bin", "vissws_recon_online")
 
 
 def _declared():
-    txt = open(HEADER).read()
+    txt = open(HEADER).read() + open(os.path.join(ROOT, "include", "vw_unet.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(vw_[a-z0-9_]+)\s*\(", txt)))
 

@@ -166,6 +166,23 @@ DEVICE_ONLY = {
     "version": (C.c_char_p, []),
 }
 
+PF = C.POINTER(C.c_float)
+
+# include/vw_unet.h (learned deconvolver)
+UNET_FUNCS = {
+    "unet_create": (C.c_int, [i32, i32, i32, C.c_uint64, C.POINTER(P)]),
+    "unet_destroy": (None, [P]),
+    "unet_last_error": (C.c_char_p, [P]),
+    "unet_param_count": (C.c_int64, [P]),
+    "unet_get_params": (C.c_int, [P, PF]),
+    "unet_set_params": (C.c_int, [P, PF]),
+    "unet_loss_grad": (C.c_int, [P, PF, PF, i32, PD, PF]),
+    "unet_set_data": (C.c_int, [P, PF, PF, i64]),
+    "unet_train": (C.c_int, [P, i64, i32, C.c_float, C.c_uint64, PF]),
+    "unet_predict": (C.c_int, [P, PF, i32, PF]),
+    "unet_train_ms": (C.c_int, [P, PD]),
+}
+
 ORACLE_ONLY = {
     "viss_half": (C.c_int, [P, i64, i64, PD, PI64]),
     "set_viss_raw": (C.c_int, [P, PD]),

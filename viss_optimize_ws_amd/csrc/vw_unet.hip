@@ -1,0 +1,852 @@
+/*
+ * vw_unet.hip -- the learned deconvolver (include/vw_unet.h): U-Net-lite
+ * forward, backward and Adam in hand-written gfx950 kernels, orchestrated
+ * by a small C++ layer table.  No counterpart in the reference.
+ *
+ * Tensors are NCHW with square images.  A 3x3 convolution workgroup computes
+ * a 16 x 16 output tile for 16 output channels of one image: each lane owns
+ * one pixel and 16 FP32 accumulators, 16 input channels at a time are staged
+ * with their 1-pixel halo in LDS (18 x 18 each), and the weights, uniform
+ * across the workgroup, come through scalar loads.  The same kernel computes
+ * the data gradient with the transposed, flipped weights.  Weight gradients
+ * reduce one (output, input) channel pair per workgroup over all pixels of
+ * the batch in a fixed order (deterministic).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/vissws.h"
+#include "../../include/vw_unet.h"
+
+#define VW_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_unet_err;
+
+/* ------------------------------------------------------------------ */
+/* kernels                                                              */
+/* ------------------------------------------------------------------ */
+
+/* 3x3 convolution, zero padding.  TR = false: out[o] = B[o] + sum_i W[o][i] * in[i]
+ * (layer weights [Co][Ci][3][3], cin = Ci, cout = Co), then ReLU if RELU.
+ * TR = true: the data gradient of such a layer, out[i] = sum_o W[o][i] (flipped) * in[o]
+ * (cin = layer Co, cout = layer Ci). */
+template <typename T, bool TR, bool RELU>
+__global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const float* __restrict__ W,
+                                               const float* __restrict__ B, T* __restrict__ out, int cin, int cout,
+                                               int H) {
+  __shared__ float tile[16][18][18];
+  const int tpr = (H + 15) / 16;
+  const int ty0 = (blockIdx.x / tpr) * 16, tx0 = (blockIdx.x % tpr) * 16;
+  const int ob = blockIdx.y * 16;
+  const int n = blockIdx.z;
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  const int64_t HH = (int64_t)H * H;
+  float acc[16];
+#pragma unroll
+  for (int o = 0; o < 16; ++o) acc[o] = 0.f;
+  for (int c0 = 0; c0 < cin; c0 += 16) {
+    const int cn = cin - c0 < 16 ? cin - c0 : 16;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cn * 324; idx += 256) {
+      const int c = idx / 324, r = idx - c * 324;
+      const int yy = r / 18, xx = r - yy * 18;
+      const int gy = ty0 + yy - 1, gx = tx0 + xx - 1;
+      float v = 0.f;
+      if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = (float)in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
+      tile[c][yy][xx] = v;
+    }
+    __syncthreads();
+    for (int c = 0; c < cn; ++c) {
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[k] = tile[c][ty + k / 3][tx + k % 3];
+#pragma unroll
+      for (int o = 0; o < 16; ++o) {
+        if (ob + o < cout) { /* uniform */
+          const float* w =
+              TR ? W + ((int64_t)(c0 + c) * cout + ob + o) * 9 : W + ((int64_t)(ob + o) * cin + c0 + c) * 9;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) acc[o] = fmaf(TR ? w[8 - k] : w[k], v[k], acc[o]);
+        }
+      }
+    }
+  }
+  const int y = ty0 + ty, x = tx0 + tx;
+  if (y >= H || x >= H) return;
+#pragma unroll
+  for (int o = 0; o < 16; ++o) {
+    if (ob + o < cout) {
+      float r = acc[o] + (B ? B[ob + o] : 0.f);
+      if (RELU) r = r > 0.f ? r : 0.f;
+      out[((int64_t)n * cout + ob + o) * HH + (int64_t)y * H + x] = (T)r;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ float vw_block_sum(float v, float* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const float r = red[0];
+  __syncthreads();
+  return r;
+}
+
+/* weight and bias gradients of a 3x3 layer: one workgroup per (ci, co) */
+template <typename T>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const T* __restrict__ in, const T* __restrict__ dz,
+                                                     float* __restrict__ dW, float* __restrict__ dB, int N, int ci_n,
+                                                     int co_n, int H) {
+  __shared__ float red[256];
+  const int ci = blockIdx.x, co = blockIdx.y;
+  const int64_t HH = (int64_t)H * H;
+  float acc[9], ab = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.f;
+  for (int64_t p = threadIdx.x; p < (int64_t)N * HH; p += 256) {
+    const int n = (int)(p / HH);
+    const int r = (int)(p - n * HH);
+    const int y = r / H, x = r - y * H;
+    const float g = (float)dz[((int64_t)n * co_n + co) * HH + r];
+    ab += g;
+    const T* src = in + ((int64_t)n * ci_n + ci) * HH;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
+      if (yy >= 0 && yy < H && xx >= 0 && xx < H) acc[k] = fmaf(g, (float)src[(int64_t)yy * H + xx], acc[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const float s = vw_block_sum<T>(acc[k], red);
+    if (threadIdx.x == 0) dW[((int64_t)co * ci_n + ci) * 9 + k] = s;
+  }
+  const float sb = vw_block_sum<T>(ab, red);
+  if (ci == 0 && threadIdx.x == 0) dB[co] = sb;
+}
+
+/* dz = dy * (y > 0) */
+template <typename T>
+__global__ void k_relu_bwd(const T* __restrict__ dy, const T* __restrict__ y, T* __restrict__ dz, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = (float)y[i] > 0.f ? dy[i] : (T)0.f;
+}
+
+/* 2x2 average pooling of [nc][H][H] to [nc][H/2][H/2] */
+template <typename T>
+__global__ void k_pool(const T* __restrict__ x, T* __restrict__ y, int64_t nc, int H) {
+  const int h = H / 2;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * h * h) return;
+  const int64_t c = i / ((int64_t)h * h);
+  const int r = (int)(i - c * h * h), yy = r / h, xx = r - yy * h;
+  const T* s = x + c * H * H + (int64_t)(2 * yy) * H + 2 * xx;
+  y[i] = (T)(0.25f * (((float)s[0] + (float)s[1]) + ((float)s[H] + (float)s[H + 1])));
+}
+/* dx = base + pool^T(dy): each fine pixel gets a quarter of its coarse gradient */
+template <typename T>
+__global__ void k_pool_bwd_add(const T* __restrict__ dy, const T* __restrict__ base, T* __restrict__ dx, int64_t nc,
+                               int H) {
+  const int h = H / 2;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * H * H) return;
+  const int64_t c = i / ((int64_t)H * H);
+  const int r = (int)(i - c * H * H), y = r / H, x = r - y * H;
+  const float g = 0.25f * (float)dy[c * h * h + (int64_t)(y / 2) * h + x / 2];
+  dx[i] = (T)((base ? (float)base[i] : 0.f) + g);
+}
+/* nearest-neighbour upsampling of [nc][h][h] to [nc][2h][2h] */
+template <typename T>
+__global__ void k_up(const T* __restrict__ x, T* __restrict__ y, int64_t nc, int h) {
+  const int H = 2 * h;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * H * H) return;
+  const int64_t c = i / ((int64_t)H * H);
+  const int r = (int)(i - c * H * H), yy = r / H, xx = r - yy * H;
+  y[i] = x[c * h * h + (int64_t)(yy / 2) * h + xx / 2];
+}
+template <typename T>
+__global__ void k_up_bwd(const T* __restrict__ dy, T* __restrict__ dx, int64_t nc, int h) {
+  const int H = 2 * h;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * h * h) return;
+  const int64_t c = i / ((int64_t)h * h);
+  const int r = (int)(i - c * h * h), yy = r / h, xx = r - yy * h;
+  const T* s = dy + c * H * H + (int64_t)(2 * yy) * H + 2 * xx;
+  dx[i] = (T)(((float)s[0] + (float)s[1]) + ((float)s[H] + (float)s[H + 1]));
+}
+template <typename T>
+__global__ void k_add(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ c, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = (T)((float)a[i] + (float)b[i]);
+}
+
+/* 1x1 output convolution plus the input: out = b + sum_c w[c] d[c] + x */
+template <typename T>
+__global__ void k_out(const T* __restrict__ d, const T* __restrict__ x, const float* __restrict__ w,
+                      const float* __restrict__ b, T* __restrict__ out, int N, int C, int64_t HH) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N * HH) return;
+  const int64_t n = i / HH, p = i - n * HH;
+  float s = b[0];
+  for (int c = 0; c < C; ++c) s = fmaf(w[c], (float)d[(n * C + c) * HH + p], s);
+  out[i] = (T)(s + (float)x[i]);
+}
+/* masked squared error: gradient dout = 2 (out - y) mask / count and
+ * per-block partial sums of the squared error */
+template <typename T>
+__global__ __launch_bounds__(256) void k_mse(const T* __restrict__ out, const T* __restrict__ y,
+                                             const uint8_t* __restrict__ mask, int64_t HH, int64_t n, float inv_count,
+                                             T* __restrict__ dout, float* __restrict__ part) {
+  __shared__ float red[256];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float e2 = 0.f;
+  if (i < n) {
+    const bool m = mask[i % HH] != 0;
+    const float e = m ? (float)out[i] - (float)y[i] : 0.f;
+    e2 = e * e;
+    dout[i] = (T)(2.f * e * inv_count);
+  }
+  const float s = vw_block_sum<T>(e2, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+/* ordered sum of the partials -> loss[slot] (FP64) */
+__global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ part, int64_t np, float inv_count,
+                                                  double* __restrict__ loss) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 256) s += (double)part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = red[0] * (double)inv_count;
+}
+/* output layer backward: dd[c] = w[c] dout, and dw[c] / db by one workgroup per c */
+template <typename T>
+__global__ void k_out_bwd_data(const T* __restrict__ dout, const float* __restrict__ w, T* __restrict__ dd, int N,
+                               int C, int64_t HH) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * C * HH) return;
+  const int64_t n = i / (C * HH), r = i - n * C * HH, c = r / HH, p = r - c * HH;
+  dd[i] = (T)(w[c] * (float)dout[n * HH + p]);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_out_wgrad(const T* __restrict__ dout, const T* __restrict__ d,
+                                                   float* __restrict__ dw, float* __restrict__ db, int N, int C,
+                                                   int64_t HH) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f, sb = 0.f;
+  for (int64_t i = threadIdx.x; i < N * HH; i += 256) {
+    const int64_t n = i / HH, p = i - n * HH;
+    const float g = (float)dout[i];
+    s = fmaf(g, (float)d[(n * C + c) * HH + p], s);
+    sb += g;
+  }
+  const float ts = vw_block_sum<T>(s, red);
+  const float tb = vw_block_sum<T>(sb, red);
+  if (threadIdx.x == 0) {
+    dw[c] = ts;
+    if (c == 0) db[0] = tb;
+  }
+}
+/* batch gather from the resident set */
+template <typename T>
+__global__ void k_gather(const T* __restrict__ X, const T* __restrict__ Y, const int32_t* __restrict__ idx, int n,
+                         int64_t HH, T* __restrict__ xb, T* __restrict__ yb) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * HH) return;
+  const int64_t b = i / HH, p = i - b * HH;
+  const int64_t src = (int64_t)idx[b] * HH + p;
+  xb[i] = X[src];
+  yb[i] = Y[src];
+}
+template <typename T>
+__global__ void k_from_f32(const float* __restrict__ a, T* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (T)a[i];
+}
+template <typename T>
+__global__ void k_to_f32(const T* __restrict__ a, float* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+/* Adam with bias correction; step t >= 1 */
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float c1, float c2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = 0.9f * m[i] + 0.1f * gi;
+  const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + 1e-8f);
+}
+
+/* ------------------------------------------------------------------ */
+/* network                                                              */
+/* ------------------------------------------------------------------ */
+
+struct Conv {
+  int ci, co, lvl;   /* channels and level (image side img_n >> lvl) */
+  int64_t w, b;      /* offsets in the parameter vector */
+};
+
+struct Net {
+  int N = 0, BM = 0;
+  bool fp16 = false;
+  std::string err;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  std::vector<Conv> conv; /* 14 3x3 layers in forward order, then the 1x1 output */
+  int64_t nparam = 0;
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  int64_t adam_t = 0;
+  uint8_t* mask = nullptr;
+  int64_t nmask = 0;
+  /* activations and gradients, each [BM][C][H][H] of the storage type */
+  enum {
+    X0, E0A, E0, P1, E1A, E1, P2, E2A, E2, P3, E3A, E3, U2, D2A, S2, D2, U1, D1A, S1, D1, U0, D0A, S0, D0, OUT, YT, NT
+  };
+  int ch[NT] = {}, lv[NT] = {};
+  void* a[NT] = {};
+  void* d[NT] = {};
+  void* dz = nullptr; /* masked gradient scratch, the largest tensor */
+  float* part = nullptr;
+  double* loss = nullptr; /* per-step losses on the device */
+  int64_t loss_cap = 0;
+  void *X = nullptr, *Y = nullptr;
+  int64_t count = 0;
+  int32_t* idx = nullptr;
+  int64_t idx_cap = 0;
+  double train_ms = 0.0;
+
+  int fail(int code, const std::string& e) {
+    err = e;
+    return code;
+  }
+  int H(int l) const { return N >> l; }
+  int64_t elems(int t) const { return (int64_t)ch[t] * H(lv[t]) * H(lv[t]); }
+};
+
+#define UC(x)                                                                              \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+template <typename P>
+int ualloc(Net* c, P** p, size_t bytes) {
+  void* q = nullptr;
+  UC(hipMalloc(&q, bytes ? bytes : 1));
+  c->allocs.push_back(q);
+  *p = (P*)q;
+  return VW_OK;
+}
+
+template <typename T>
+struct Ops {
+  static int conv3(Net* c, const Conv& L, int src, int dst, int n) {
+    const int H = c->H(L.lvl);
+    const int tpr = (H + 15) / 16;
+    dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.co + 15) / 16), (unsigned)n);
+    k_conv3<T, false, true><<<grid, 256, 0, c->st>>>((const T*)c->a[src], c->w + L.w, c->w + L.b, (T*)c->a[dst], L.ci,
+                                                     L.co, H);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+  /* backward of layer L whose input is tensor src and output tensor dst with
+   * gradient d[dst]: weight gradients, and (if dsrc >= 0) d[dsrc] */
+  static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n) {
+    const int H = c->H(L.lvl);
+    const int64_t ne = (int64_t)n * c->elems(dst);
+    k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
+    UC(hipGetLastError());
+    k_conv3_wgrad<T><<<dim3((unsigned)L.ci, (unsigned)L.co), 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz,
+                                                                             c->g + L.w, c->g + L.b, n, L.ci, L.co, H);
+    UC(hipGetLastError());
+    if (dsrc >= 0) {
+      const int tpr = (H + 15) / 16;
+      dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.ci + 15) / 16), (unsigned)n);
+      k_conv3<T, true, false><<<grid, 256, 0, c->st>>>((const T*)c->dz, c->w + L.w, nullptr, (T*)c->d[dsrc], L.co,
+                                                       L.ci, H);
+      UC(hipGetLastError());
+    }
+    return VW_OK;
+  }
+  static int pool(Net* c, int s, int t, int n) {
+    const int64_t nc = (int64_t)n * c->ch[s];
+    const int H = c->H(c->lv[s]);
+    k_pool<T><<<nblk(nc * (H / 2) * (H / 2)), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, H);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+  static int up(Net* c, int s, int t, int n) {
+    const int64_t nc = (int64_t)n * c->ch[s];
+    const int h = c->H(c->lv[s]);
+    k_up<T><<<nblk(nc * 4 * h * h), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, h);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+  static int add(Net* c, int x, int y, int z, int n) {
+    const int64_t ne = (int64_t)n * c->elems(z);
+    k_add<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->a[x], (const T*)c->a[y], (T*)c->a[z], ne);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+
+  static int forward(Net* c, int n) {
+    const std::vector<Conv>& L = c->conv;
+    int rc;
+#define F(x) if ((rc = (x))) return rc
+    F(conv3(c, L[0], Net::X0, Net::E0A, n));
+    F(conv3(c, L[1], Net::E0A, Net::E0, n));
+    F(pool(c, Net::E0, Net::P1, n));
+    F(conv3(c, L[2], Net::P1, Net::E1A, n));
+    F(conv3(c, L[3], Net::E1A, Net::E1, n));
+    F(pool(c, Net::E1, Net::P2, n));
+    F(conv3(c, L[4], Net::P2, Net::E2A, n));
+    F(conv3(c, L[5], Net::E2A, Net::E2, n));
+    F(pool(c, Net::E2, Net::P3, n));
+    F(conv3(c, L[6], Net::P3, Net::E3A, n));
+    F(conv3(c, L[7], Net::E3A, Net::E3, n));
+    F(up(c, Net::E3, Net::U2, n));
+    F(conv3(c, L[8], Net::U2, Net::D2A, n));
+    F(add(c, Net::D2A, Net::E2, Net::S2, n));
+    F(conv3(c, L[9], Net::S2, Net::D2, n));
+    F(up(c, Net::D2, Net::U1, n));
+    F(conv3(c, L[10], Net::U1, Net::D1A, n));
+    F(add(c, Net::D1A, Net::E1, Net::S1, n));
+    F(conv3(c, L[11], Net::S1, Net::D1, n));
+    F(up(c, Net::D1, Net::U0, n));
+    F(conv3(c, L[12], Net::U0, Net::D0A, n));
+    F(add(c, Net::D0A, Net::E0, Net::S0, n));
+    F(conv3(c, L[13], Net::S0, Net::D0, n));
+    const Conv& O = L[14];
+    const int64_t HH = (int64_t)c->N * c->N;
+    k_out<T><<<nblk(n * HH), 256, 0, c->st>>>((const T*)c->a[Net::D0], (const T*)c->a[Net::X0], c->w + O.w,
+                                              c->w + O.b, (T*)c->a[Net::OUT], n, 16, HH);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+
+  /* loss into loss[slot], gradients into c->g */
+  static int backward(Net* c, int n, int64_t slot) {
+    const std::vector<Conv>& L = c->conv;
+    const int64_t HH = (int64_t)c->N * c->N;
+    const int64_t ne = n * HH;
+    const unsigned nb = nblk(ne);
+    const float inv = 1.0f / (float)(c->nmask * n);
+    k_mse<T><<<nb, 256, 0, c->st>>>((const T*)c->a[Net::OUT], (const T*)c->a[Net::YT], c->mask, HH, ne, inv,
+                                    (T*)c->d[Net::OUT], c->part);
+    UC(hipGetLastError());
+    k_loss_sum<<<1, 256, 0, c->st>>>(c->part, nb, inv, c->loss + slot);
+    UC(hipGetLastError());
+    const Conv& O = L[14];
+    k_out_wgrad<T><<<16, 256, 0, c->st>>>((const T*)c->d[Net::OUT], (const T*)c->a[Net::D0], c->g + O.w, c->g + O.b,
+                                          n, 16, HH);
+    UC(hipGetLastError());
+    k_out_bwd_data<T><<<nblk(ne * 16), 256, 0, c->st>>>((const T*)c->d[Net::OUT], c->w + O.w, (T*)c->d[Net::D0], n,
+                                                        16, HH);
+    UC(hipGetLastError());
+    int rc;
+    /* decoder level 0 */
+    F(conv3_bwd(c, L[13], Net::S0, Net::D0, Net::S0, n));            /* dS0 */
+    F(copy(c, Net::S0, Net::D0A, n));                                /* dD0A = dS0 */
+    F(conv3_bwd(c, L[12], Net::U0, Net::D0A, Net::U0, n));           /* dU0 */
+    F(up_bwd(c, Net::U0, Net::D1, n));                               /* dD1 */
+    F(conv3_bwd(c, L[11], Net::S1, Net::D1, Net::S1, n));
+    F(copy(c, Net::S1, Net::D1A, n));
+    F(conv3_bwd(c, L[10], Net::U1, Net::D1A, Net::U1, n));
+    F(up_bwd(c, Net::U1, Net::D2, n));
+    F(conv3_bwd(c, L[9], Net::S2, Net::D2, Net::S2, n));
+    F(copy(c, Net::S2, Net::D2A, n));
+    F(conv3_bwd(c, L[8], Net::U2, Net::D2A, Net::U2, n));
+    F(up_bwd(c, Net::U2, Net::E3, n));                               /* dE3 */
+    /* encoder, deepest first; the skip gradients dS* seed dE0..dE2 */
+    F(conv3_bwd(c, L[7], Net::E3A, Net::E3, Net::E3A, n));
+    F(conv3_bwd(c, L[6], Net::P3, Net::E3A, Net::P3, n));
+    F(pool_bwd(c, Net::P3, Net::S2, Net::E2, n));                    /* dE2 = dS2 + pool^T dP3 */
+    F(conv3_bwd(c, L[5], Net::E2A, Net::E2, Net::E2A, n));
+    F(conv3_bwd(c, L[4], Net::P2, Net::E2A, Net::P2, n));
+    F(pool_bwd(c, Net::P2, Net::S1, Net::E1, n));
+    F(conv3_bwd(c, L[3], Net::E1A, Net::E1, Net::E1A, n));
+    F(conv3_bwd(c, L[2], Net::P1, Net::E1A, Net::P1, n));
+    F(pool_bwd(c, Net::P1, Net::S0, Net::E0, n));
+    F(conv3_bwd(c, L[1], Net::E0A, Net::E0, Net::E0A, n));
+    F(conv3_bwd(c, L[0], Net::X0, Net::E0A, -1, n));
+#undef F
+    return VW_OK;
+  }
+  static int copy(Net* c, int s, int t, int n) {
+    UC(hipMemcpyAsync(c->d[t], c->d[s], (size_t)(n * c->elems(s)) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    return VW_OK;
+  }
+  static int up_bwd(Net* c, int s_up, int t, int n) {
+    const int64_t nc = (int64_t)n * c->ch[t];
+    const int h = c->H(c->lv[t]);
+    k_up_bwd<T><<<nblk(nc * h * h), 256, 0, c->st>>>((const T*)c->d[s_up], (T*)c->d[t], nc, h);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+  /* d[t] = d[base] + pool^T d[s] (t, base at the fine level) */
+  static int pool_bwd(Net* c, int s, int base, int t, int n) {
+    const int64_t nc = (int64_t)n * c->ch[t];
+    const int H = c->H(c->lv[t]);
+    k_pool_bwd_add<T><<<nblk(nc * H * H), 256, 0, c->st>>>((const T*)c->d[s], (const T*)c->d[base], (T*)c->d[t], nc,
+                                                           H);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+  static int upload(Net* c, const float* host, int t, int n, bool grad = false) {
+    const int64_t ne = n * c->elems(t);
+    float* tmp = nullptr;
+    UC(hipMalloc(&tmp, sizeof(float) * ne));
+    hipError_t e = hipMemcpyAsync(tmp, host, sizeof(float) * ne, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) {
+      k_from_f32<T><<<nblk(ne), 256, 0, c->st>>>(tmp, (T*)(grad ? c->d[t] : c->a[t]), ne);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
+    return VW_OK;
+  }
+  static int download(Net* c, int t, int n, float* host) {
+    const int64_t ne = n * c->elems(t);
+    float* tmp = nullptr;
+    UC(hipMalloc(&tmp, sizeof(float) * ne));
+    k_to_f32<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->a[t], tmp, ne);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, sizeof(float) * ne, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
+    return VW_OK;
+  }
+  static int gather(Net* c, const int32_t* idx, int n) {
+    const int64_t HH = (int64_t)c->N * c->N;
+    k_gather<T><<<nblk(n * HH), 256, 0, c->st>>>((const T*)c->X, (const T*)c->Y, idx, n, HH, (T*)c->a[Net::X0],
+                                                 (T*)c->a[Net::YT]);
+    UC(hipGetLastError());
+    return VW_OK;
+  }
+};
+
+int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
+  c->N = img_n;
+  c->BM = bm;
+  c->fp16 = fp16;
+  const int cs[4] = {16, 32, 64, 128};
+  /* (ci, co, level) of the 14 3x3 layers in forward order, then the 1x1 */
+  const int spec[15][3] = {{1, 16, 0},   {16, 16, 0},  {16, 32, 1},  {32, 32, 1},  {32, 64, 2},
+                           {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {128, 64, 2}, {64, 64, 2},
+                           {64, 32, 1},  {32, 32, 1},  {32, 16, 0},  {16, 16, 0},  {16, 1, 0}};
+  (void)cs;
+  int64_t off = 0;
+  for (int k = 0; k < 15; ++k) {
+    Conv L;
+    L.ci = spec[k][0];
+    L.co = spec[k][1];
+    L.lvl = spec[k][2];
+    const int ks = k < 14 ? 9 : 1;
+    L.w = off;
+    off += (int64_t)L.co * L.ci * ks;
+    L.b = off;
+    off += L.co;
+    c->conv.push_back(L);
+  }
+  c->nparam = off;
+  /* tensor channels and levels */
+  const int tc[Net::NT][2] = {{1, 0},  {16, 0},  {16, 0},  {16, 1}, {32, 1}, {32, 1}, {32, 2}, {64, 2}, {64, 2},
+                              {64, 3}, {128, 3}, {128, 3}, {128, 2}, {64, 2}, {64, 2}, {64, 2}, {64, 1}, {32, 1},
+                              {32, 1}, {32, 1},  {32, 0},  {16, 0}, {16, 0}, {16, 0}, {1, 0},  {1, 0}};
+  for (int t = 0; t < Net::NT; ++t) {
+    c->ch[t] = tc[t][0];
+    c->lv[t] = tc[t][1];
+  }
+  hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("stream: ") + hipGetErrorString(e));
+  const size_t ts = fp16 ? 2 : 4;
+  int rc;
+  int64_t maxe = 0;
+  for (int t = 0; t < Net::NT; ++t) {
+    const size_t bytes = (size_t)bm * c->elems(t) * ts;
+    maxe = std::max<int64_t>(maxe, (int64_t)bm * c->elems(t));
+    if ((rc = ualloc(c, &c->a[t], bytes)) || (rc = ualloc(c, &c->d[t], bytes))) return rc;
+  }
+  if ((rc = ualloc(c, &c->dz, (size_t)maxe * ts))) return rc;
+  const int64_t HH = (int64_t)img_n * img_n;
+  if ((rc = ualloc(c, &c->w, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->g, sizeof(float) * c->nparam)) ||
+      (rc = ualloc(c, &c->m, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->v, sizeof(float) * c->nparam)) ||
+      (rc = ualloc(c, &c->mask, (size_t)HH)) || (rc = ualloc(c, &c->part, sizeof(float) * (bm * HH / 256 + 1))))
+    return rc;
+  /* lm-grid disk: pixel centres (2j + 1 - N) / N, inside when l^2 + m^2 < 1 */
+  std::vector<uint8_t> mk((size_t)HH);
+  c->nmask = 0;
+  for (int i = 0; i < img_n; ++i)
+    for (int j = 0; j < img_n; ++j) {
+      const int64_t a = 2 * j + 1 - img_n, b = 2 * i + 1 - img_n;
+      mk[(size_t)i * img_n + j] = (a * a + b * b < (int64_t)img_n * img_n) ? 1 : 0;
+      c->nmask += mk[(size_t)i * img_n + j];
+    }
+  UC(hipMemcpy(c->mask, mk.data(), (size_t)HH, hipMemcpyHostToDevice));
+  /* He-normal weights, zero biases; the 1x1 output layer starts at zero so
+   * the untrained network returns its input */
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  std::vector<float> w((size_t)c->nparam, 0.f);
+  for (int k = 0; k < 14; ++k) {
+    const Conv& L = c->conv[k];
+    const double sd = std::sqrt(2.0 / (9.0 * L.ci));
+    for (int64_t i = 0; i < (int64_t)L.co * L.ci * 9; ++i) w[(size_t)(L.w + i)] = (float)(sd * nd(rng));
+  }
+  UC(hipMemcpy(c->w, w.data(), sizeof(float) * c->nparam, hipMemcpyHostToDevice));
+  UC(hipMemset(c->m, 0, sizeof(float) * c->nparam));
+  UC(hipMemset(c->v, 0, sizeof(float) * c->nparam));
+  return VW_OK;
+}
+
+Net* as_net(void* p) { return (Net*)p; }
+
+template <typename F>
+int dispatch(Net* c, F&& f) {
+  return c->fp16 ? f(Ops<_Float16>{}) : f(Ops<float>{});
+}
+
+}  // namespace
+
+VW_API int vw_unet_create(int32_t img_n, int32_t batch_max, int32_t act_fp16, uint64_t seed, void** out) {
+  if (!out) return VW_ERR_ARG;
+  *out = nullptr;
+  if (img_n < 8 || img_n % 8 != 0 || img_n > 4096 || batch_max < 1 || batch_max > 4096) {
+    g_unet_err = "img_n must be a multiple of 8 in [8, 4096] and batch_max in [1, 4096]";
+    return VW_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_unet_err = "no HIP device";
+    return VW_ERR_DEVICE;
+  }
+  Net* c = new Net();
+  int rc = build(c, img_n, batch_max, act_fp16 != 0, seed);
+  if (rc) {
+    g_unet_err = c->err;
+    vw_unet_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return VW_OK;
+}
+
+VW_API void vw_unet_destroy(void* p) {
+  Net* c = as_net(p);
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void* q : c->allocs) (void)hipFree(q);
+  if (c->X) (void)hipFree(c->X);
+  if (c->Y) (void)hipFree(c->Y);
+  if (c->idx) (void)hipFree(c->idx);
+  if (c->loss) (void)hipFree(c->loss);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+VW_API const char* vw_unet_last_error(void* p) { return p ? as_net(p)->err.c_str() : g_unet_err.c_str(); }
+
+VW_API int64_t vw_unet_param_count(void* p) { return p ? as_net(p)->nparam : -1; }
+
+VW_API int vw_unet_get_params(void* p, float* out) {
+  Net* c = as_net(p);
+  if (!c || !out) return VW_ERR_ARG;
+  UC(hipMemcpy(out, c->w, sizeof(float) * c->nparam, hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_unet_set_params(void* p, const float* in) {
+  Net* c = as_net(p);
+  if (!c || !in) return VW_ERR_ARG;
+  UC(hipMemcpy(c->w, in, sizeof(float) * c->nparam, hipMemcpyHostToDevice));
+  return VW_OK;
+}
+
+namespace {
+int ensure_loss(Net* c, int64_t n) {
+  if (n <= c->loss_cap) return VW_OK;
+  if (c->loss) (void)hipFree(c->loss);
+  c->loss = nullptr;
+  c->loss_cap = 0;
+  UC(hipMalloc(&c->loss, sizeof(double) * n));
+  c->loss_cap = n;
+  return VW_OK;
+}
+}  // namespace
+
+VW_API int vw_unet_loss_grad(void* p, const float* X, const float* Y, int32_t n, double* loss, float* grad) {
+  Net* c = as_net(p);
+  if (!c || !X || !Y) return VW_ERR_ARG;
+  if (n < 1 || n > c->BM) return c->fail(VW_ERR_ARG, "n must be in [1, batch_max]");
+  int rc;
+  if ((rc = ensure_loss(c, 1))) return rc;
+  rc = dispatch(c, [&](auto ops) {
+    using O = decltype(ops);
+    int r;
+    if ((r = O::upload(c, X, Net::X0, n)) || (r = O::upload(c, Y, Net::YT, n)) || (r = O::forward(c, n)) ||
+        (r = O::backward(c, n, 0)))
+      return r;
+    return VW_OK;
+  });
+  if (rc) return rc;
+  UC(hipStreamSynchronize(c->st));
+  if (loss) UC(hipMemcpy(loss, c->loss, sizeof(double), hipMemcpyDeviceToHost));
+  if (grad) UC(hipMemcpy(grad, c->g, sizeof(float) * c->nparam, hipMemcpyDeviceToHost));
+  return VW_OK;
+}
+
+VW_API int vw_unet_set_data(void* p, const float* X, const float* Y, int64_t count) {
+  Net* c = as_net(p);
+  if (!c || !X || !Y || count < 1) return VW_ERR_ARG;
+  const int64_t HH = (int64_t)c->N * c->N;
+  const size_t ts = c->fp16 ? 2 : 4;
+  if (c->X) (void)hipFree(c->X);
+  if (c->Y) (void)hipFree(c->Y);
+  c->X = c->Y = nullptr;
+  c->count = 0;
+  UC(hipMalloc(&c->X, (size_t)(count * HH) * ts));
+  UC(hipMalloc(&c->Y, (size_t)(count * HH) * ts));
+  /* convert in slices through a device FP32 staging buffer */
+  const int64_t slice = std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)(64 << 20) / HH));
+  float* tmp = nullptr;
+  UC(hipMalloc(&tmp, sizeof(float) * slice * HH));
+  hipError_t e = hipSuccess;
+  for (int64_t s0 = 0; s0 < count && e == hipSuccess; s0 += slice) {
+    const int64_t k = std::min(slice, count - s0);
+    for (int which = 0; which < 2 && e == hipSuccess; ++which) {
+      e = hipMemcpy(tmp, (which ? Y : X) + s0 * HH, sizeof(float) * k * HH, hipMemcpyHostToDevice);
+      if (e != hipSuccess) break;
+      void* dst = which ? c->Y : c->X;
+      if (c->fp16)
+        k_from_f32<_Float16><<<nblk(k * HH), 256, 0, nullptr>>>(tmp, (_Float16*)dst + s0 * HH, k * HH);
+      else
+        k_from_f32<float><<<nblk(k * HH), 256, 0, nullptr>>>(tmp, (float*)dst + s0 * HH, k * HH);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+  }
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("set_data: ") + hipGetErrorString(e));
+  c->count = count;
+  return VW_OK;
+}
+
+VW_API int vw_unet_train(void* p, int64_t steps, int32_t batch, float lr, uint64_t seed, float* losses) {
+  Net* c = as_net(p);
+  if (!c || steps < 1 || !(lr > 0.f)) return VW_ERR_ARG;
+  if (batch < 1 || batch > c->BM) return c->fail(VW_ERR_ARG, "batch must be in [1, batch_max]");
+  if (c->count < batch) return c->fail(VW_ERR_STATE, "vw_unet_train needs vw_unet_set_data with at least one batch");
+  /* batch indices for every step: a fresh shuffle of the set per epoch */
+  std::vector<int32_t> idx((size_t)(steps * batch));
+  std::vector<int32_t> perm((size_t)c->count);
+  for (int64_t i = 0; i < c->count; ++i) perm[(size_t)i] = (int32_t)i;
+  std::mt19937_64 rng(seed);
+  int64_t pos = c->count;
+  for (int64_t s = 0; s < steps * batch; ++s) {
+    if (pos + 1 > c->count) {
+      for (int64_t i = c->count - 1; i > 0; --i) {
+        const int64_t j = (int64_t)(rng() % (uint64_t)(i + 1));
+        std::swap(perm[(size_t)i], perm[(size_t)j]);
+      }
+      pos = 0;
+    }
+    idx[(size_t)s] = perm[(size_t)pos++];
+  }
+  if (steps * batch > c->idx_cap) {
+    if (c->idx) (void)hipFree(c->idx);
+    c->idx = nullptr;
+    c->idx_cap = 0;
+    UC(hipMalloc(&c->idx, sizeof(int32_t) * steps * batch));
+    c->idx_cap = steps * batch;
+  }
+  UC(hipMemcpy(c->idx, idx.data(), sizeof(int32_t) * steps * batch, hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = ensure_loss(c, steps))) return rc;
+  hipEvent_t e0, e1;
+  UC(hipEventCreate(&e0));
+  UC(hipEventCreate(&e1));
+  UC(hipEventRecord(e0, c->st));
+  for (int64_t s = 0; s < steps; ++s) {
+    rc = dispatch(c, [&](auto ops) {
+      using O = decltype(ops);
+      int r;
+      if ((r = O::gather(c, c->idx + s * batch, batch)) || (r = O::forward(c, batch)) || (r = O::backward(c, batch, s)))
+        return r;
+      return VW_OK;
+    });
+    if (rc) break;
+    ++c->adam_t;
+    const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)c->adam_t));
+    const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)c->adam_t));
+    k_adam<<<nblk(c->nparam), 256, 0, c->st>>>(c->w, c->g, c->m, c->v, c->nparam, lr, c1, c2);
+    if (hipGetLastError() != hipSuccess) {
+      rc = c->fail(VW_ERR_DEVICE, "adam launch");
+      break;
+    }
+  }
+  (void)hipEventRecord(e1, c->st);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  c->train_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (losses) {
+    std::vector<double> l((size_t)steps);
+    UC(hipMemcpy(l.data(), c->loss, sizeof(double) * steps, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < steps; ++s) losses[s] = (float)l[(size_t)s];
+  }
+  return VW_OK;
+}
+
+VW_API int vw_unet_predict(void* p, const float* X, int32_t n, float* out) {
+  Net* c = as_net(p);
+  if (!c || !X || !out || n < 0) return VW_ERR_ARG;
+  const int64_t HH = (int64_t)c->N * c->N;
+  for (int32_t b0 = 0; b0 < n; b0 += c->BM) {
+    const int k = std::min(c->BM, n - b0);
+    int rc = dispatch(c, [&](auto ops) {
+      using O = decltype(ops);
+      int r;
+      if ((r = O::upload(c, X + b0 * HH, Net::X0, k)) || (r = O::forward(c, k)) || (r = O::download(c, Net::OUT, k, out + b0 * HH)))
+        return r;
+      return VW_OK;
+    });
+    if (rc) return rc;
+  }
+  return VW_OK;
+}
+
+VW_API int vw_unet_train_ms(void* p, double* ms) {
+  Net* c = as_net(p);
+  if (!c || !ms) return VW_ERR_ARG;
+  *ms = c->train_ms;
+  return VW_OK;
+}

@@ -37,6 +37,7 @@ def load_library():
         lib = C.CDLL(p)
         A.bind(lib, "vw_", A.STAGE_FUNCS)
         A.bind(lib, "vw_", A.DEVICE_ONLY)
+        A.bind(lib, "vw_", A.UNET_FUNCS)
         _LIB = lib
     return _LIB
 
