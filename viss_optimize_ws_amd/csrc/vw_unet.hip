@@ -104,18 +104,22 @@ __device__ __forceinline__ float vw_block_sum(float v, float* red) {
   return r;
 }
 
-/* weight and bias gradients of a 3x3 layer: one workgroup per (ci, co) */
+/* weight and bias gradients of a 3x3 layer: one workgroup per (ci, co) and
+ * chunk of pixels (blockIdx.z), partial sums [chunk][co][ci][10] (9 weights,
+ * then the bias term) folded in chunk order by k_wgrad_fold */
+#define VW_WG_CHUNK 16384
 template <typename T>
 __global__ __launch_bounds__(256) void k_conv3_wgrad(const T* __restrict__ in, const T* __restrict__ dz,
-                                                     float* __restrict__ dW, float* __restrict__ dB, int N, int ci_n,
-                                                     int co_n, int H) {
+                                                     float* __restrict__ part, int N, int ci_n, int co_n, int H) {
   __shared__ float red[256];
   const int ci = blockIdx.x, co = blockIdx.y;
   const int64_t HH = (int64_t)H * H;
+  const int64_t p0 = (int64_t)blockIdx.z * VW_WG_CHUNK;
+  const int64_t p1 = p0 + VW_WG_CHUNK < (int64_t)N * HH ? p0 + VW_WG_CHUNK : (int64_t)N * HH;
   float acc[9], ab = 0.f;
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = 0.f;
-  for (int64_t p = threadIdx.x; p < (int64_t)N * HH; p += 256) {
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
     const int n = (int)(p / HH);
     const int r = (int)(p - n * HH);
     const int y = r / H, x = r - y * H;
@@ -128,13 +132,27 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const T* __restrict__ in, c
       if (yy >= 0 && yy < H && xx >= 0 && xx < H) acc[k] = fmaf(g, (float)src[(int64_t)yy * H + xx], acc[k]);
     }
   }
+  float* dst = part + (((int64_t)blockIdx.z * co_n + co) * ci_n + ci) * 10;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     const float s = vw_block_sum<T>(acc[k], red);
-    if (threadIdx.x == 0) dW[((int64_t)co * ci_n + ci) * 9 + k] = s;
+    if (threadIdx.x == 0) dst[k] = s;
   }
   const float sb = vw_block_sum<T>(ab, red);
-  if (ci == 0 && threadIdx.x == 0) dB[co] = sb;
+  if (threadIdx.x == 0) dst[9] = sb;
+}
+/* chunk partials -> dW [co][ci][9] and dB [co] (the ci = 0 partials), in chunk order */
+__global__ void k_wgrad_fold(const float* __restrict__ part, int nchunk, int co_n, int ci_n, float* __restrict__ dW,
+                             float* __restrict__ dB) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)co_n * ci_n * 10;
+  if (i >= per) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[c * per + i];
+  const int64_t pair = i / 10, k = i - pair * 10;
+  const int64_t co = pair / ci_n, ci = pair - co * ci_n;
+  if (k < 9) dW[(co * ci_n + ci) * 9 + k] = s;
+  else if (ci == 0) dB[co] = s;
 }
 
 /* dz = dy * (y > 0) */
@@ -204,11 +222,15 @@ __global__ void k_out(const T* __restrict__ d, const T* __restrict__ x, const fl
   for (int c = 0; c < C; ++c) s = fmaf(w[c], (float)d[(n * C + c) * HH + p], s);
   out[i] = (T)(s + (float)x[i]);
 }
-/* masked squared error: gradient dout = 2 (out - y) mask / count and
- * per-block partial sums of the squared error */
+/* masked squared error: per-block partial sums of the squared error and the
+ * scaled output gradient dout = 2 (out - y) mask * gscale.  With gscale =
+ * 1 / batch (not 1 / (batch * disk pixels)) the gradients stay O(1) per
+ * pixel, clear of the FP16 subnormal range; every parameter gradient is then
+ * the loss gradient times the disk's pixel count, which Adam's update does
+ * not see and vw_unet_loss_grad divides out. */
 template <typename T>
 __global__ __launch_bounds__(256) void k_mse(const T* __restrict__ out, const T* __restrict__ y,
-                                             const uint8_t* __restrict__ mask, int64_t HH, int64_t n, float inv_count,
+                                             const uint8_t* __restrict__ mask, int64_t HH, int64_t n, float gscale,
                                              T* __restrict__ dout, float* __restrict__ part) {
   __shared__ float red[256];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,7 +239,7 @@ __global__ __launch_bounds__(256) void k_mse(const T* __restrict__ out, const T*
     const bool m = mask[i % HH] != 0;
     const float e = m ? (float)out[i] - (float)y[i] : 0.f;
     e2 = e * e;
-    dout[i] = (T)(2.f * e * inv_count);
+    dout[i] = (T)(2.f * e * gscale);
   }
   const float s = vw_block_sum<T>(e2, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -245,14 +267,16 @@ __global__ void k_out_bwd_data(const T* __restrict__ dout, const float* __restri
   const int64_t n = i / (C * HH), r = i - n * C * HH, c = r / HH, p = r - c * HH;
   dd[i] = (T)(w[c] * (float)dout[n * HH + p]);
 }
+/* one workgroup per (channel, chunk of pixels); partials [chunk][C][2] */
 template <typename T>
 __global__ __launch_bounds__(256) void k_out_wgrad(const T* __restrict__ dout, const T* __restrict__ d,
-                                                   float* __restrict__ dw, float* __restrict__ db, int N, int C,
-                                                   int64_t HH) {
+                                                   float* __restrict__ part, int N, int C, int64_t HH) {
   __shared__ float red[256];
   const int c = blockIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.y * VW_WG_CHUNK;
+  const int64_t p1 = p0 + VW_WG_CHUNK < N * HH ? p0 + VW_WG_CHUNK : N * HH;
   float s = 0.f, sb = 0.f;
-  for (int64_t i = threadIdx.x; i < N * HH; i += 256) {
+  for (int64_t i = p0 + threadIdx.x; i < p1; i += 256) {
     const int64_t n = i / HH, p = i - n * HH;
     const float g = (float)dout[i];
     s = fmaf(g, (float)d[(n * C + c) * HH + p], s);
@@ -261,9 +285,21 @@ __global__ __launch_bounds__(256) void k_out_wgrad(const T* __restrict__ dout, c
   const float ts = vw_block_sum<T>(s, red);
   const float tb = vw_block_sum<T>(sb, red);
   if (threadIdx.x == 0) {
-    dw[c] = ts;
-    if (c == 0) db[0] = tb;
+    part[((int64_t)blockIdx.y * C + c) * 2] = ts;
+    part[((int64_t)blockIdx.y * C + c) * 2 + 1] = tb;
   }
+}
+__global__ void k_out_wgrad_fold(const float* __restrict__ part, int nchunk, int C, float* __restrict__ dw,
+                                 float* __restrict__ db) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f, sb = 0.f;
+  for (int k = 0; k < nchunk; ++k) {
+    s += part[((int64_t)k * C + c) * 2];
+    sb += part[((int64_t)k * C + c) * 2 + 1];
+  }
+  dw[c] = s;
+  if (c == 0) db[0] = sb;
 }
 /* batch gather from the resident set */
 template <typename T>
@@ -329,6 +365,7 @@ struct Net {
   void* d[NT] = {};
   void* dz = nullptr; /* masked gradient scratch, the largest tensor */
   float* part = nullptr;
+  float* wpart = nullptr; /* weight-gradient chunk partials */
   double* loss = nullptr; /* per-step losses on the device */
   int64_t loss_cap = 0;
   void *X = nullptr, *Y = nullptr;
@@ -380,8 +417,12 @@ struct Ops {
     const int64_t ne = (int64_t)n * c->elems(dst);
     k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
-    k_conv3_wgrad<T><<<dim3((unsigned)L.ci, (unsigned)L.co), 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz,
-                                                                             c->g + L.w, c->g + L.b, n, L.ci, L.co, H);
+    const int nch = (int)((ne / L.co + VW_WG_CHUNK - 1) / VW_WG_CHUNK);
+    k_conv3_wgrad<T><<<dim3((unsigned)L.ci, (unsigned)L.co, (unsigned)nch), 256, 0, c->st>>>(
+        (const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
+    UC(hipGetLastError());
+    k_wgrad_fold<<<nblk((int64_t)L.co * L.ci * 10), 256, 0, c->st>>>(c->wpart, nch, L.co, L.ci, c->g + L.w,
+                                                                     c->g + L.b);
     UC(hipGetLastError());
     if (dsrc >= 0) {
       const int tpr = (H + 15) / 16;
@@ -455,14 +496,17 @@ struct Ops {
     const int64_t ne = n * HH;
     const unsigned nb = nblk(ne);
     const float inv = 1.0f / (float)(c->nmask * n);
-    k_mse<T><<<nb, 256, 0, c->st>>>((const T*)c->a[Net::OUT], (const T*)c->a[Net::YT], c->mask, HH, ne, inv,
-                                    (T*)c->d[Net::OUT], c->part);
+    k_mse<T><<<nb, 256, 0, c->st>>>((const T*)c->a[Net::OUT], (const T*)c->a[Net::YT], c->mask, HH, ne,
+                                    1.0f / (float)n, (T*)c->d[Net::OUT], c->part);
     UC(hipGetLastError());
     k_loss_sum<<<1, 256, 0, c->st>>>(c->part, nb, inv, c->loss + slot);
     UC(hipGetLastError());
     const Conv& O = L[14];
-    k_out_wgrad<T><<<16, 256, 0, c->st>>>((const T*)c->d[Net::OUT], (const T*)c->a[Net::D0], c->g + O.w, c->g + O.b,
-                                          n, 16, HH);
+    const int onch = (int)((ne + VW_WG_CHUNK - 1) / VW_WG_CHUNK);
+    k_out_wgrad<T><<<dim3(16, (unsigned)onch), 256, 0, c->st>>>((const T*)c->d[Net::OUT], (const T*)c->a[Net::D0],
+                                                                c->wpart, n, 16, HH);
+    UC(hipGetLastError());
+    k_out_wgrad_fold<<<1, 64, 0, c->st>>>(c->wpart, onch, 16, c->g + O.w, c->g + O.b);
     UC(hipGetLastError());
     k_out_bwd_data<T><<<nblk(ne * 16), 256, 0, c->st>>>((const T*)c->d[Net::OUT], c->w + O.w, (T*)c->d[Net::D0], n,
                                                         16, HH);
@@ -594,6 +638,19 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
     if ((rc = ualloc(c, &c->a[t], bytes)) || (rc = ualloc(c, &c->d[t], bytes))) return rc;
   }
   if ((rc = ualloc(c, &c->dz, (size_t)maxe * ts))) return rc;
+  /* weight-gradient partials: the largest chunks x pairs x 10 of any layer */
+  {
+    int64_t wp = 1;
+    const int spec_l[14][3] = {{1, 16, 0},   {16, 16, 0},  {16, 32, 1},  {32, 32, 1},  {32, 64, 2},
+                               {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {128, 64, 2}, {64, 64, 2},
+                               {64, 32, 1},  {32, 32, 1},  {32, 16, 0},  {16, 16, 0}};
+    for (int k = 0; k < 14; ++k) {
+      const int64_t px = (int64_t)bm * (img_n >> spec_l[k][2]) * (img_n >> spec_l[k][2]);
+      const int64_t nch = (px + VW_WG_CHUNK - 1) / VW_WG_CHUNK;
+      wp = std::max<int64_t>(wp, nch * spec_l[k][0] * spec_l[k][1] * 10);
+    }
+    if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp))) return rc;
+  }
   const int64_t HH = (int64_t)img_n * img_n;
   if ((rc = ualloc(c, &c->w, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->g, sizeof(float) * c->nparam)) ||
       (rc = ualloc(c, &c->m, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->v, sizeof(float) * c->nparam)) ||
@@ -717,7 +774,11 @@ VW_API int vw_unet_loss_grad(void* p, const float* X, const float* Y, int32_t n,
   if (rc) return rc;
   UC(hipStreamSynchronize(c->st));
   if (loss) UC(hipMemcpy(loss, c->loss, sizeof(double), hipMemcpyDeviceToHost));
-  if (grad) UC(hipMemcpy(grad, c->g, sizeof(float) * c->nparam, hipMemcpyDeviceToHost));
+  if (grad) {
+    UC(hipMemcpy(grad, c->g, sizeof(float) * c->nparam, hipMemcpyDeviceToHost));
+    const float un = 1.0f / (float)c->nmask; /* the backward pass works at gscale = 1 / n */
+    for (int64_t i = 0; i < c->nparam; ++i) grad[i] *= un;
+  }
   return VW_OK;
 }
 

@@ -145,9 +145,9 @@ def front_sky(seed, npix, n_dir):
     return np.where(n_dir > 0.0, make_sky(seed, npix, 1), 0.0)
 
 
-def make_pairs(seeds, nside=64, img_n=256, cfg=None):
-    """(dirty images, target skies on the grid, HEALPix skies) for the seeds,
-    through the plain FP64 pipeline"""
+def make_pairs(seeds, nside=64, img_n=256, cfg=None, log=None):
+    """(dirty images, target skies on the grid) for the seeds, through the
+    plain FP64 pipeline"""
     cfg = cfg or geometry_config(nside, img_n)
     idx = grid_pixel_index(nside, img_n)
     X = np.zeros((len(seeds), img_n, img_n), np.float32)
@@ -162,6 +162,8 @@ def make_pairs(seeds, nside=64, img_n=256, cfg=None):
             p.run_all()
             X[k] = p.finalize()[2]
             Y[k] = np.where(idx >= 0, sky[np.maximum(idx, 0)], 0.0)
+            if log and (k + 1) % 2000 == 0:
+                log(f"{k + 1} pairs")
     return X, Y
 
 
@@ -252,7 +254,7 @@ def run(pairs=20000, heldout=512, nside=64, img_n=256, batch=16, epochs=2.0, lr=
     t0 = time.perf_counter()
     seeds = np.arange(100000, 100000 + pairs)
     hseeds = np.arange(900000, 900000 + heldout)
-    X, Y = make_pairs(seeds, nside, img_n)
+    X, Y = make_pairs(seeds, nside, img_n, log=log)
     Xh, Yh = make_pairs(hseeds, nside, img_n)
     t_data = time.perf_counter() - t0
     if log:
@@ -263,8 +265,17 @@ def run(pairs=20000, heldout=512, nside=64, img_n=256, batch=16, epochs=2.0, lr=
     with UNet(img_n, batch, fp16=fp16, seed=seed) as net:
         net.set_data(x, y)
         del x, y
-        losses = net.train(steps, batch, lr, seed=seed)
-        train_ms = net.train_ms()
+        # ten slices of the step budget (Adam state carries over), one log line each
+        losses, train_ms = [], 0.0
+        for i in range(10):
+            k = steps // 10 + (1 if i < steps % 10 else 0)
+            if k == 0:
+                continue
+            losses.append(net.train(k, batch, lr, seed=seed * 1000 + i))
+            train_ms += net.train_ms()
+            if log:
+                log(f"train slice {i + 1}/10: loss {float(losses[-1][-1]):.4g}")
+        losses = np.concatenate(losses)
         est = denormalise(net.predict(xh), sh, gain)
     e_net = disk_rel_err(est, Yh)
     e_dirty = disk_rel_err(best_scaled(Xh, Yh), Yh)
