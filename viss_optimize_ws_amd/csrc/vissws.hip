@@ -137,6 +137,7 @@ struct Ctx {
   void* d_otmp[2] = {};
   size_t otmp_bytes[2] = {};
   double2* d_orec = nullptr; /* (ugu, vgu) of the group at each work-order position */
+  double* d_oant = nullptr;  /* its representative's antenna frames [position][10]    */
   double *d_Cacc = nullptr, *d_Gacc = nullptr;
   uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
   double *d_out = nullptr, *d_iout = nullptr;
@@ -591,7 +592,8 @@ int create_batch(Ctx* c) {
     if ((rc = dalloc(c, &tmp, c->otmp_bytes[w]))) return rc;
     c->d_otmp[w] = tmp;
   }
-  if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad))) return rc;
+  if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad)) || (rc = dalloc(c, &c->d_oant, 10 * (size_t)c->nq_pad)))
+    return rc;
   if ((rc = dalloc(c, &c->d_brpart, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart, (size_t)wp))) return rc;
   if (c->ngp > 0 && ((rc = dalloc(c, &c->d_brpart2, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart2, (size_t)wp))))
     return rc;
@@ -1011,7 +1013,8 @@ int enq_group(Ctx* c) {
   if (c->batch) {
     int rc;
     if ((rc = enq_order(c, 1))) return rc;
-    k_group_order_recs<<<blocks_for(c->nq_pad), 256, 0, c->stB>>>(c->d_order[1], c->gs.rec, c->nq_pad, c->d_orec);
+    k_group_order_recs<<<blocks_for(c->nq_pad), 256, 0, c->stB>>>(c->d_order[1], c->gs.rec, c->gs.ant, c->nq_pad,
+                                                                  c->d_orec, c->d_oant);
     KC();
     k_group_pack<<<blocks_for(c->nq_pad * c->nsky_pad), 256, 0, c->stB>>>(
         c->d_goff, c->d_idx_sorted, c->d_vh, c->nh, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode, c->d_info,
@@ -1056,7 +1059,7 @@ int enq_recon_set_batch(Ctx* c, bool img) {
     a.m = img ? c->d_gm : c->d_m;
     a.n = img ? c->d_gn : c->d_n;
     a.pt0 = pt0; a.npts = np; a.TP = c->TP;
-    a.gcls = cls; a.nq_pad = c->nq_pad; a.gperm = c->d_order[1]; a.orec = c->d_orec;
+    a.gcls = cls; a.nq_pad = c->nq_pad; a.gperm = c->d_order[1]; a.orec = c->d_orec; a.oant = c->d_oant;
     a.rec = c->gs.rec; a.gant = c->gs.ant; a.info = c->d_info;
     a.vrh = (const uint4*)c->d_gpk[0]; a.vrl = (const uint4*)c->d_gpk[1];
     a.vih = (const uint4*)c->d_gpk[2]; a.vil = (const uint4*)c->d_gpk[3];

@@ -135,13 +135,17 @@ __global__ void k_row_keys(const RowRec* __restrict__ rows, const double* __rest
   keys[i] = i < nh ? vw_pair_key(ant8, rows[i].ia, rows[i].ib) : 4096;
   idx[i] = (int32_t)i;
 }
-/* cell centres in the recon's work order */
-__global__ void k_group_order_recs(const int32_t* __restrict__ order, const GRec* __restrict__ rec, int64_t n,
-                                   double2* __restrict__ orec) {
+/* cell centres and representative antenna frames in the recon's work order */
+__global__ void k_group_order_recs(const int32_t* __restrict__ order, const GRec* __restrict__ rec,
+                                   const double* __restrict__ gant, int64_t n, double2* __restrict__ orec,
+                                   double* __restrict__ oant) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const GRec g = rec[order[j]];
+  const int64_t q = order[j];
+  const GRec g = rec[q];
   orec[j] = make_double2(g.ugu, g.vgu);
+#pragma unroll
+  for (int f = 0; f < 10; ++f) oant[j * 10 + f] = gant[q * 10 + f];
 }
 __global__ void k_group_keys(const int32_t* __restrict__ gia, const int32_t* __restrict__ gib,
                              const double* __restrict__ ant8, const DevInfo* __restrict__ info, int64_t n,
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
   __syncthreads();
 
   int cls = VW_CLS_HIDDEN;
-  bool act = false, any_mixed = false;
+  bool act = false, any_mixed = false, frames = false;
   double ax = 0, ay = 0, az = 0, ai = 0, ac = 0, bx = 0, by = 0, bz = 0, bi = 0, bc = 0;
   for (int64_t k = 0; k < K; ++k) {
     const int64_t sidx = k >> 3;
@@ -295,11 +299,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
       cls = wave_on ? (int)a.cls[(tile - a.tile0) * a.nh_pad + h] : VW_CLS_HIDDEN;
       act = __ballot(cls != VW_CLS_HIDDEN) != 0;
       any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
-      if (any_mixed && cls == VW_CLS_MIXED) {
+      if (any_mixed && cls == VW_CLS_MIXED && !frames) {
+        /* the row's antenna frames: loaded once, at its first mixed sub-tile */
         const double* qa = a.ant8 + (int64_t)rec.ia * 8;
         const double* qb = a.ant8 + (int64_t)rec.ib * 8;
         ax = qa[0]; ay = qa[1]; az = qa[2]; ai = qa[3]; ac = qa[7];
         bx = qb[0]; by = qb[1]; bz = qb[2]; bi = qb[3]; bc = qb[7];
+        frames = true;
       }
     }
     if (act) {
@@ -465,6 +471,7 @@ struct ReconBArgs {
   const uint8_t* gcls;        /* [tiles of the pass][nq_pad] classes in work order  */
   const int32_t* gperm;       /* work order: position j is group gperm[j]          */
   const double2* orec;        /* (ugu, vgu) in work order                          */
+  const double* oant;         /* [position][10] representative frames, work order  */
   int64_t nq_pad;
   const GRec* rec;            /* cell centres (wavelengths)                        */
   const double* gant;         /* [q][10] representative antenna frames             */
@@ -492,6 +499,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
   __shared__ __attribute__((aligned(16))) __bf16 bsh[2][Stage::ROWS * VW_LROW];
   __shared__ double2 grec[2][32];                      /* (ugu, vgu)      */
   __shared__ int32_t gq[2][32];                        /* group index     */
+  __shared__ double gfr[2][320];                       /* frames, 10 per position */
   __shared__ __attribute__((aligned(16))) uint8_t gcl[2][32]; /* classes */
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t pbw = vb % a.pbw;
@@ -525,6 +533,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
    * centre (ugu, vgu); 32-33: 16 class bytes of the workgroup's tile; the
    * records and the class table are laid out in work order */
   double2 xr = make_double2(0.0, 0.0);
+  double xf0 = 0.0, xf1 = 0.0; /* frame doubles t and t + 256 of the k-step (t + 256 < 320) */
   int32_t xq = 0;
   vw_u32x4 xc = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
 #define VW_X_LOAD(k_)                                                                    \
@@ -538,6 +547,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
     } else if (t_ < 34 && a.rep) {                                                       \
       xc = *(const vw_u32x4*)(a.gcls + tile * a.nq_pad + j0_ + 16 * (t_ - 32));          \
     }                                                                                    \
+    if (a.rep) {                                                                         \
+      xf0 = a.oant[j0_ * 10 + t_];                                                       \
+      if (t_ < 64) xf1 = a.oant[j0_ * 10 + 256 + t_];                                    \
+    }                                                                                    \
   } while (0)
 #define VW_X_STORE(b_)                                        \
   do {                                                        \
@@ -547,6 +560,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
       gq[b_][t_] = xq;                                        \
     } else if (t_ < 34) {                                     \
       *(vw_u32x4*)(&gcl[b_][16 * (t_ - 32)]) = xc;            \
+    }                                                         \
+    if (a.rep) {                                              \
+      gfr[b_][t_] = xf0;                                      \
+      if (t_ < 64) gfr[b_][256 + t_] = xf1;                   \
     }                                                         \
   } while (0)
   vw_f32x4b acc[NB];
@@ -588,7 +605,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
         const int cls = (int)((cw >> (8 * i)) & 0xffu);
         bool on = jk + i < qe && q >= q0 && cls != VW_CLS_HIDDEN;
         if (on && cls == VW_CLS_MIXED) {
-          const double* d = a.gant + q * 10;
+          const double* d = &gfr[buf][(g * 8 + i) * 10];
           on = vw_visible(pl, pm, pn, d[0], d[1], d[2], d[3], d[4]) &&
                vw_visible(pl, pm, pn, d[5], d[6], d[7], d[8], d[9]);
         }
