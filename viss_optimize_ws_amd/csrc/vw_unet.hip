@@ -9,8 +9,9 @@
  * with their 1-pixel halo in LDS (18 x 18 each), and the weights, uniform
  * across the workgroup, come through scalar loads.  The same kernel computes
  * the data gradient with the transposed, flipped weights.  Weight gradients
- * reduce one (output, input) channel pair per workgroup over all pixels of
- * the batch in a fixed order (deterministic).
+ * are register-blocked over 16 x 16 channel pairs per workgroup and summed
+ * over pixel chunks whose partials are folded in a fixed order
+ * (deterministic).
  */
 #include <hip/hip_runtime.h>
 
@@ -104,43 +105,146 @@ __device__ __forceinline__ float vw_block_sum(float v, float* red) {
   return r;
 }
 
-/* weight and bias gradients of a 3x3 layer: one workgroup per (ci, co) and
- * chunk of pixels (blockIdx.z), partial sums [chunk][co][ci][10] (9 weights,
- * then the bias term) folded in chunk order by k_wgrad_fold */
+/* pixels per chunk of the output layer's weight-gradient reduction */
 #define VW_WG_CHUNK 16384
+
+/* weight and bias gradients of a 3x3 layer: a workgroup owns 16 output x 16
+ * input channels and one chunk of 16 x 16 pixel tiles; per tile it stages dz
+ * of its output channels (16 x 16) and the input channels with their halo
+ * (18 x 18) in LDS, and each lane accumulates 2 output x 2 input channels x
+ * 9 taps (8 LDS reads per 36 FMAs), over a quarter of the tile's rows.
+ * Partials [chunk][co][ci][10] (9 weights, then the bias term from the
+ * ci = 0 lanes), folded in chunk order by k_wgrad_fold. */
+#define VW_WG_TILES 16 /* 16 x 16 pixel tiles per chunk */
 template <typename T>
-__global__ __launch_bounds__(256) void k_conv3_wgrad(const T* __restrict__ in, const T* __restrict__ dz,
-                                                     float* __restrict__ part, int N, int ci_n, int co_n, int H) {
-  __shared__ float red[256];
-  const int ci = blockIdx.x, co = blockIdx.y;
+__global__ __launch_bounds__(256) void k_conv3_wgrad_t(const T* __restrict__ in, const T* __restrict__ dz,
+                                                       float* __restrict__ part, int N, int ci_n, int co_n, int H) {
+  /* one arena: the staged tiles (16 x 18 x 18 inputs, 16 x 16 x 17 gradients)
+   * and, at the end, the quarter partials (3 x 64 lanes x 38) */
+  __shared__ float arena[16 * 324 + 16 * 16 * 17];
+  static_assert(3 * 64 * 38 <= 16 * 324 + 16 * 16 * 17, "reduction area");
+  float (*sx)[18][18] = reinterpret_cast<float (*)[18][18]>(arena);
+  float (*sg)[16][17] = reinterpret_cast<float (*)[16][17]>(arena + 16 * 324);
+  const int cib = blockIdx.x * 16, cob = blockIdx.y * 16;
+  const int tpr = (H + 15) / 16;
+  const int64_t ntiles = (int64_t)N * tpr * tpr;
+  const int64_t t0 = (int64_t)blockIdx.z * VW_WG_TILES;
+  const int64_t t1 = t0 + VW_WG_TILES < ntiles ? t0 + VW_WG_TILES : ntiles;
+  const int tid = threadIdx.x;
+  const int oi = (tid >> 3) & 7, ii = tid & 7;  /* lane's channel pairs: co cob + 2 oi + {0,1}, ci cib + 2 ii + {0,1} */
+  const int half = tid >> 6;                     /* 0..3: which quarter of the tile's rows the lane sums */
   const int64_t HH = (int64_t)H * H;
-  const int64_t p0 = (int64_t)blockIdx.z * VW_WG_CHUNK;
-  const int64_t p1 = p0 + VW_WG_CHUNK < (int64_t)N * HH ? p0 + VW_WG_CHUNK : (int64_t)N * HH;
-  float acc[9], ab = 0.f;
+  float acc[2][2][9], ab[2] = {0.f, 0.f};
 #pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k] = 0.f;
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    const int n = (int)(p / HH);
-    const int r = (int)(p - n * HH);
-    const int y = r / H, x = r - y * H;
-    const float g = (float)dz[((int64_t)n * co_n + co) * HH + r];
-    ab += g;
-    const T* src = in + ((int64_t)n * ci_n + ci) * HH;
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
-      if (yy >= 0 && yy < H && xx >= 0 && xx < H) acc[k] = fmaf(g, (float)src[(int64_t)yy * H + xx], acc[k]);
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc[a][b][k] = 0.f;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int n = (int)(t / (tpr * tpr));
+    const int r = (int)(t - (int64_t)n * tpr * tpr);
+    const int y0 = (r / tpr) * 16, x0 = (r % tpr) * 16;
+    __syncthreads();
+    for (int idx = tid; idx < 16 * 324; idx += 256) {
+      const int c = idx / 324, q = idx - c * 324, yy = q / 18, xx = q - yy * 18;
+      const int gy = y0 + yy - 1, gx = x0 + xx - 1;
+      float v = 0.f;
+      if (cib + c < ci_n && gy >= 0 && gy < H && gx >= 0 && gx < H)
+        v = (float)in[((int64_t)n * ci_n + cib + c) * HH + (int64_t)gy * H + gx];
+      sx[c][yy][xx] = v;
+    }
+    for (int idx = tid; idx < 16 * 256; idx += 256) {
+      const int c = idx >> 8, q = idx & 255, yy = q >> 4, xx = q & 15;
+      const int gy = y0 + yy, gx = x0 + xx;
+      float v = 0.f;
+      if (cob + c < co_n && gy < H && gx < H) v = (float)dz[((int64_t)n * co_n + cob + c) * HH + (int64_t)gy * H + gx];
+      sg[c][yy][xx] = v;
+    }
+    __syncthreads();
+    /* the lane sums rows 4 half .. 4 half + 3 of the tile (the four quarters
+     * are added at the end), every column */
+#pragma unroll 1
+    for (int yy = 4 * half; yy < 4 * half + 4; ++yy) {
+      float w0[2][3], w1[2][3], w2[2][3]; /* 3 x 3 input window rows for the two input channels */
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int kx = 0; kx < 2; ++kx) {
+          w0[b][kx + 1] = sx[2 * ii + b][yy][kx];
+          w1[b][kx + 1] = sx[2 * ii + b][yy + 1][kx];
+          w2[b][kx + 1] = sx[2 * ii + b][yy + 2][kx];
+        }
+#pragma unroll 2
+      for (int xx = 0; xx < 16; ++xx) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          w0[b][0] = w0[b][1]; w0[b][1] = w0[b][2]; w0[b][2] = sx[2 * ii + b][yy][xx + 2];
+          w1[b][0] = w1[b][1]; w1[b][1] = w1[b][2]; w1[b][2] = sx[2 * ii + b][yy + 1][xx + 2];
+          w2[b][0] = w2[b][1]; w2[b][1] = w2[b][2]; w2[b][2] = sx[2 * ii + b][yy + 2][xx + 2];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const float gv = sg[2 * oi + a][yy][xx];
+          if (ii == 0) ab[a] += gv;
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+              acc[a][b][kx] = fmaf(gv, w0[b][kx], acc[a][b][kx]);
+              acc[a][b][3 + kx] = fmaf(gv, w1[b][kx], acc[a][b][3 + kx]);
+              acc[a][b][6 + kx] = fmaf(gv, w2[b][kx], acc[a][b][6 + kx]);
+            }
+          }
+        }
+      }
     }
   }
-  float* dst = part + (((int64_t)blockIdx.z * co_n + co) * ci_n + ci) * 10;
+  /* fold the four row quarters through the arena, lanes of quarter 0 write */
+  __syncthreads();
+  float* red = arena;
+  const int lane64 = tid & 63;
+  if (half > 0) {
+    float* d = red + ((half - 1) * 64 + lane64) * 38;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const float s = vw_block_sum<T>(acc[k], red);
-    if (threadIdx.x == 0) dst[k] = s;
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d[(a * 2 + b) * 9 + k] = acc[a][b][k];
+    d[36] = ab[0];
+    d[37] = ab[1];
   }
-  const float sb = vw_block_sum<T>(ab, red);
-  if (threadIdx.x == 0) dst[9] = sb;
+  __syncthreads();
+  if (half == 0) {
+    for (int h = 0; h < 3; ++h) {
+      const float* d = red + (h * 64 + lane64) * 38;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) acc[a][b][k] += d[(a * 2 + b) * 9 + k];
+      ab[0] += d[36];
+      ab[1] += d[37];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int co = cob + 2 * oi + a;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int ci = cib + 2 * ii + b;
+        if (co < co_n && ci < ci_n) {
+          float* dst = part + (((int64_t)blockIdx.z * co_n + co) * ci_n + ci) * 10;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) dst[k] = acc[a][b][k];
+          if (ci == 0) dst[9] = ab[a];
+        }
+      }
+    }
+  }
 }
+
 /* chunk partials -> dW [co][ci][9] and dB [co] (the ci = 0 partials), in chunk order */
 __global__ void k_wgrad_fold(const float* __restrict__ part, int nchunk, int co_n, int ci_n, float* __restrict__ dW,
                              float* __restrict__ dB) {
@@ -417,9 +521,10 @@ struct Ops {
     const int64_t ne = (int64_t)n * c->elems(dst);
     k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
-    const int nch = (int)((ne / L.co + VW_WG_CHUNK - 1) / VW_WG_CHUNK);
-    k_conv3_wgrad<T><<<dim3((unsigned)L.ci, (unsigned)L.co, (unsigned)nch), 256, 0, c->st>>>(
-        (const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
+    const int tpr = (H + 15) / 16;
+    const int nch = (int)(((int64_t)n * tpr * tpr + VW_WG_TILES - 1) / VW_WG_TILES);
+    k_conv3_wgrad_t<T><<<dim3((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)nch), 256, 0,
+                         c->st>>>((const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
     UC(hipGetLastError());
     k_wgrad_fold<<<nblk((int64_t)L.co * L.ci * 10), 256, 0, c->st>>>(c->wpart, nch, L.co, L.ci, c->g + L.w,
                                                                      c->g + L.b);
@@ -645,9 +750,12 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
                                {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {128, 64, 2}, {64, 64, 2},
                                {64, 32, 1},  {32, 32, 1},  {32, 16, 0},  {16, 16, 0}};
     for (int k = 0; k < 14; ++k) {
-      const int64_t px = (int64_t)bm * (img_n >> spec_l[k][2]) * (img_n >> spec_l[k][2]);
-      const int64_t nch = (px + VW_WG_CHUNK - 1) / VW_WG_CHUNK;
+      const int64_t side = img_n >> spec_l[k][2];
+      const int64_t tiles = (int64_t)bm * ((side + 15) / 16) * ((side + 15) / 16);
+      const int64_t nch = (tiles + VW_WG_TILES - 1) / VW_WG_TILES;
       wp = std::max<int64_t>(wp, nch * spec_l[k][0] * spec_l[k][1] * 10);
+      /* the output layer's partials: pixel chunks x 16 channels x 2 */
+      wp = std::max<int64_t>(wp, ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32);
     }
     if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp))) return rc;
   }
