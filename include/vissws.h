@@ -254,6 +254,14 @@ int vw_get_batch_targets(void* ctx, int64_t* viss_waves, int64_t* recon_waves);
  * accumulators are cleared afterwards.  Needs vw_set_sky. */
 int vw_autotune(void* ctx, int32_t reps, int64_t* viss_waves, int64_t* recon_waves);
 
+/* One batched step as a graph: clears the accumulators and runs every
+ * segment of the schedule (vw_reset_accumulators + vw_run_all), from a HIP
+ * graph captured on the first call and replayed afterwards (recaptured after
+ * vw_set_batch_targets, vw_autotune or vw_set_day).  Batched contexts only;
+ * the step uses no memset and no library sort.  Follow with vw_finalize.
+ * No counterpart in the reference. */
+int vw_run_all_graph(void* ctx);
+
 /* ---------------- introspection (tests) ---------------- */
 
 /* Resident pixel directions: l, m, n, nm1 (npix each). */

@@ -72,6 +72,9 @@ CASES = {
     "tp512_uneven": dict(nside=32, n_ant=12, n_epochs=40, img_n=48, nsky=40, tile_pix=512, seg_tlen=[5, 19, 16]),
     # a small class budget: several tile passes per stage, one block per wave
     "tile_passes": dict(nside=32, n_ant=8, n_epochs=16, segs=2, img_n=48, nsky=16, class_budget_bytes=64 * 1024),
+    # a segment of 8,400 rows: grouping by the batched contexts' own radix sort
+    # (the one-sky path sorts with rocPRIM; equal weights mean equal groups)
+    "long_segment": dict(nside=16, n_ant=16, n_epochs=72, img_n=32, nsky=20, seg_tlen=[70, 2]),
 }
 
 
@@ -174,3 +177,42 @@ def test_batch_autotune_and_launch_targets():
     with Pipeline(config(nside=8, n_ant=4, n_epochs=4, segs=1, img_n=0)) as one:
         with pytest.raises(VissError):
             one.autotune()
+
+
+def test_batch_graph_replays():
+    """a captured step replayed three times, with eager steps on another sky
+    in between, equals the eager step bit for bit every time; a change of
+    launch targets recaptures"""
+    kw = dict(nside=16, n_ant=16, n_epochs=72, seg_tlen=[70, 2], img_n=32, precision=1, nsky=20)
+    npix = 12 * 16 * 16
+    A = _skies(20, npix, 61)
+    B = _skies(20, npix, 62)
+    with Pipeline(config(**kw)) as g:
+        g.set_sky(B)
+        g.reset_accumulators()
+        g.run_all()
+        refB = g.finalize()
+        g.set_sky(A)
+        g.reset_accumulators()
+        g.run_all()
+        refA = g.finalize()
+        for _ in range(3):
+            g.set_sky(A)
+            g.run_all_graph()
+            out = g.finalize()
+            for a, b in zip(out, refA):
+                assert np.array_equal(a, b)
+            g.set_sky(B)
+            g.reset_accumulators()
+            g.run_all()
+            for a, b in zip(g.finalize(), refB):
+                assert np.array_equal(a, b)
+        g.set_batch_targets(4096, 8192)
+        g.set_sky(A)
+        g.run_all_graph()
+        out = g.finalize()
+        assert np.array_equal(out[1], refA[1]) and rel_err(out[0], refA[0]) <= MIXED_TOL
+    with Pipeline(config(nside=8, n_ant=4, n_epochs=4, segs=1, img_n=0)) as one:
+        one.set_sky(np.ones(12 * 64))
+        with pytest.raises(VissError):
+            one.run_all_graph()

@@ -131,13 +131,15 @@ struct Ctx {
   uint32_t *d_bwpart = nullptr, *d_bwpart2 = nullptr;
   int64_t brpart_cap = 0, bwpart_cap = 0;
   /* work orders of the batched kernels: keys, indices, sorted keys, order
-   * (rows on the Viss stream, groups on the grouping stream) and their
-   * radix-sort scratch */
+   * (rows on the Viss stream, groups on the grouping stream) */
   int32_t *d_okey[2] = {}, *d_oidx[2] = {}, *d_okey2[2] = {}, *d_order[2] = {};
-  void* d_otmp[2] = {};
-  size_t otmp_bytes[2] = {};
   double2* d_orec = nullptr; /* (ugu, vgu) of the group at each work-order position */
   double* d_oant = nullptr;  /* its representative's antenna frames [position][10]    */
+  int32_t* d_rshist[2] = {};  /* radix-sort digit tables per stream (Viss, grouping)   */
+  int64_t rshist_cap = 0;
+  /* graph of a whole batched step (reset + every segment), captured on first use */
+  hipGraphExec_t graph_exec = nullptr;
+  bool capturing = false;
   double *d_Cacc = nullptr, *d_Gacc = nullptr;
   uint32_t *d_Wacc = nullptr, *d_GWacc = nullptr;
   double *d_out = nullptr, *d_iout = nullptr;
@@ -180,6 +182,7 @@ inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs 
 
 /* event bracket for one stage launch group */
 hipEvent_t ev_begin(Ctx* c, Stage s, hipStream_t stream = nullptr) {
+  if (c->capturing) return nullptr; /* no timing inside a captured graph */
   auto& v = c->ev[s];
   if (c->ev_used[s] == v.size()) {
     EvPair p;
@@ -192,6 +195,7 @@ hipEvent_t ev_begin(Ctx* c, Stage s, hipStream_t stream = nullptr) {
   return p.b;
 }
 void ev_end(Ctx* c, Stage s, hipStream_t stream = nullptr) {
+  if (c->capturing) return;
   (void)hipEventRecord(c->ev[s][c->ev_used[s]].b, stream ? stream : c->st);
   c->ev_used[s]++;
 }
@@ -540,6 +544,12 @@ void select_seg(Ctx* c, int b) {
   c->d_vfull = g.vfull;
 }
 
+/* a captured step is rebuilt on its next use */
+void drop_graph(Ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+}
+
 /* the main stream waits for every recon enqueued so far */
 int join_recon(Ctx* c) {
   for (int b = 0; b < 2; ++b) HC(hipStreamWaitEvent(c->st, c->ev_recon[b], 0));
@@ -585,15 +595,15 @@ int create_batch(Ctx* c) {
     if ((rc = dalloc(c, &c->d_okey[w], (size_t)n)) || (rc = dalloc(c, &c->d_oidx[w], (size_t)n)) ||
         (rc = dalloc(c, &c->d_okey2[w], (size_t)n)) || (rc = dalloc(c, &c->d_order[w], (size_t)n)))
       return rc;
-    HC(rocprim::radix_sort_pairs(nullptr, c->otmp_bytes[w], (const int32_t*)nullptr, (int32_t*)nullptr,
-                                 (const int32_t*)nullptr, (int32_t*)nullptr, (unsigned)n, 0u,
-                                 (unsigned)VW_ORDER_BITS, c->st));
-    uint8_t* tmp = nullptr;
-    if ((rc = dalloc(c, &tmp, c->otmp_bytes[w]))) return rc;
-    c->d_otmp[w] = tmp;
   }
   if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad)) || (rc = dalloc(c, &c->d_oant, 10 * (size_t)c->nq_pad)))
     return rc;
+  {
+    const int64_t nmax = std::max<int64_t>(c->nq_pad, 2 * c->max_nh);
+    c->rshist_cap = 256 * ((nmax + VW_RS_BLOCK - 1) / VW_RS_BLOCK);
+    for (int w = 0; w < 2; ++w)
+      if ((rc = dalloc(c, &c->d_rshist[w], (size_t)c->rshist_cap))) return rc;
+  }
   if ((rc = dalloc(c, &c->d_brpart, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart, (size_t)wp))) return rc;
   if (c->ngp > 0 && ((rc = dalloc(c, &c->d_brpart2, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart2, (size_t)wp))))
     return rc;
@@ -844,30 +854,46 @@ int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
 /* work order of the batched kernels (which = 0: the segment's half rows on
  * the Viss stream; 1: its groups on the grouping stream): a stable radix
  * sort of the antenna-direction pair keys */
-#ifndef VW_GROUP_ORDER
-#define VW_GROUP_ORDER 1
-#endif
+/* stable LSD radix sort of n (key, value) pairs over `passes` 8-bit digits
+ * from bit 0 with the memset-free kernels of vw_batch.h; the pairs move
+ * between (k0, v0) and (k1, v1) and the sorted pairs end in (k1, v1) */
+template <typename K, typename V>
+int enq_rsort(Ctx* c, int which, hipStream_t st, K* k0, V* v0, K* k1, V* v1, int64_t n, int passes) {
+  const int64_t nblk = (n + VW_RS_BLOCK - 1) / VW_RS_BLOCK;
+  if (256 * nblk > c->rshist_cap) return c->fail(VW_ERR_RANGE, "internal: radix-sort table too small");
+  int32_t* h = c->d_rshist[which];
+  K* ka[2] = {k0, k1};
+  V* va[2] = {v0, v1};
+  int cur = 0;
+  for (int p = 0; p < passes; ++p, cur ^= 1) {
+    k_rs_hist<K><<<(unsigned)nblk, 256, 0, st>>>(ka[cur], n, 8 * p, h, nblk);
+    KC();
+    k_rs_scan<<<1, 1024, 0, st>>>(h, 256 * nblk);
+    KC();
+    k_rs_scatter<K, V><<<(unsigned)nblk, 256, 0, st>>>(ka[cur], va[cur], n, 8 * p, h, nblk, ka[cur ^ 1], va[cur ^ 1]);
+    KC();
+  }
+  if (cur == 0) { /* an even number of passes ends in (k0, v0) */
+    k_copy<K><<<blocks_for(n), 256, 0, st>>>(k0, k1, n);
+    KC();
+    k_copy<V><<<blocks_for(n), 256, 0, st>>>(v0, v1, n);
+    KC();
+  }
+  return VW_OK;
+}
+
 int enq_order(Ctx* c, int which) {
   hipStream_t st = which == 0 ? c->st : c->stB;
   const int64_t n = which == 0 ? c->nh_pad : c->nq_pad;
-  if (which == 1 && !VW_GROUP_ORDER) {
-    /* identity order */
-    k_group_keys<<<blocks_for(n), 256, 0, st>>>(c->gs.gia, c->gs.gib, c->d_ant8, c->d_info, n, c->d_okey[1],
-                                                c->d_order[1]);
-    KC();
-    return VW_OK;
-  }
   if (which == 0)
     k_row_keys<<<blocks_for(n), 256, 0, st>>>(c->d_rows, c->d_ant8, c->nh, n, c->d_okey[0], c->d_oidx[0]);
   else
     k_group_keys<<<blocks_for(n), 256, 0, st>>>(c->gs.gia, c->gs.gib, c->d_ant8, c->d_info, n, c->d_okey[1],
                                                 c->d_oidx[1]);
   KC();
-  size_t tb = c->otmp_bytes[which];
-  HC(rocprim::radix_sort_pairs(c->d_otmp[which], tb, (const int32_t*)c->d_okey[which], c->d_okey2[which],
-                               (const int32_t*)c->d_oidx[which], c->d_order[which], (unsigned)n, 0u,
-                               (unsigned)VW_ORDER_BITS, st));
-  return VW_OK;
+  /* keys below 2^13: two 8-bit digits */
+  return enq_rsort<int32_t, int32_t>(c, which, st, c->d_okey[which], c->d_oidx[which], c->d_okey2[which],
+                                     c->d_order[which], n, 2);
 }
 
 /* one tile pass of the batched Viss: classes, the product kernel, the fold */
@@ -984,16 +1010,24 @@ int enq_group(Ctx* c) {
   k_keys<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
   if (nr > VW_RANK_MAX) {
-    /* long segments: stable LSD radix sort of (key, row index) pairs */
+    /* long segments: stable LSD radix sort of (key, row index) pairs; batched
+     * contexts use the memset-free sort so that their steps can be captured */
     k_iota<<<blocks_for(nr), 256, 0, c->stB>>>(nr, c->d_rank);
     KC();
-    size_t tb = c->sort_tmp_bytes;
-    HC(rocprim::radix_sort_pairs(c->d_sort_tmp, tb, (const int64_t*)c->d_keys, c->d_keys_sorted,
-                                 (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->stB));
+    if (c->batch) {
+      int rc = enq_rsort<int64_t, int32_t>(c, 1, c->stB, c->d_keys, c->d_rank, c->d_keys_sorted, c->d_idx_sorted, nr,
+                                           8);
+      if (rc) return rc;
+    } else {
+      size_t tb = c->sort_tmp_bytes;
+      HC(rocprim::radix_sort_pairs(c->d_sort_tmp, tb, (const int64_t*)c->d_keys, c->d_keys_sorted,
+                                   (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->stB));
+    }
   } else {
     /* stable rank by (key, row): a grid of (256 rows) x (jblk rows) compare
      * tiles sized for about 2,048 workgroups, then a scatter */
-    HC(hipMemsetAsync(c->d_rank, 0, sizeof(int32_t) * nr, c->stB));
+    k_fill0<<<blocks_for(nr), 256, 0, c->stB>>>((uint32_t*)c->d_rank, nr);
+    KC();
     const int64_t gx = blocks_for(nr);
     const int64_t gy = std::max<int64_t>(1, std::min<int64_t>(gx, (2048 + gx - 1) / gx));
     const int64_t jblk = ((nr + gy - 1) / gy + 255) / 256 * 256;
@@ -1269,6 +1303,7 @@ VW_API void vw_destroy(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (void* q : c->allocs) (void)hipFree(q);
   for (int s = 0; s < ST_N; ++s)
     for (auto& e : c->ev[s]) {
@@ -1324,6 +1359,7 @@ VW_API int vw_set_day(void* p, int32_t day) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
   c->cfg.day = day;
+  drop_graph(c); /* bl_max and the day's scalars are captured by value */
   int rc = upload_day(c);
   if (rc) return rc;
   c->have_seg = c->have_viss = c->have_conj = c->have_groups = false;
@@ -1434,6 +1470,69 @@ VW_API int vw_run_all(void* p) {
       return rc;
   }
   if ((rc = join_recon(c))) return rc;
+  return sync(c);
+}
+
+namespace {
+/* accumulators and the running key total cleared by kernels (no memset nodes) */
+int enq_reset_fill(Ctx* c) {
+  const int64_t nC = 2 * c->npix * c->nsky; /* doubles as 32-bit words */
+  k_fill0<<<blocks_for(nC), 256, 0, c->st>>>((uint32_t*)c->d_Cacc, nC);
+  KC();
+  k_fill0<<<blocks_for(c->npix), 256, 0, c->st>>>(c->d_Wacc, c->npix);
+  KC();
+  if (c->ngp) {
+    const int64_t nG = 2 * c->ngp * c->nsky;
+    k_fill0<<<blocks_for(nG), 256, 0, c->st>>>((uint32_t*)c->d_Gacc, nG);
+    KC();
+    k_fill0<<<blocks_for(c->ngp), 256, 0, c->st>>>(c->d_GWacc, c->ngp);
+    KC();
+  }
+  k_fill0<<<1, 256, 0, c->st>>>((uint32_t*)c->d_info, (int64_t)(sizeof(DevInfo) / 4));
+  KC();
+  return VW_OK;
+}
+}  // namespace
+
+/* One batched step from a graph: clear the accumulators, then every segment of
+ * the schedule (as vw_reset_accumulators + vw_run_all).  The graph is captured
+ * on the first call and replayed afterwards; a change of launch targets or of
+ * day captures it again. */
+VW_API int vw_run_all_graph(void* p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->batch) return c->fail(VW_ERR_ARG, "graph steps are built for batched contexts (nsky > 1)");
+  if (!c->have_sky) return c->fail(VW_ERR_STATE, "vw_run_all_graph needs vw_set_sky");
+  int rc;
+  if (!c->graph_exec) {
+    if ((rc = join_recon(c)) || (rc = sync(c))) return rc;
+    HC(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    /* the waits on the recon events of the first two segments refer to records
+     * inside the capture */
+    for (int b = 0; b < 2; ++b) (void)hipEventRecord(c->ev_recon[b], c->st);
+    rc = enq_reset_fill(c);
+    for (int sg = 0; sg < c->segs && !rc; ++sg) {
+      if ((rc = enq_gen(c, c->seg_t0[sg], c->seg_tlen[sg])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c)) ||
+          (rc = enq_recon(c)))
+        break;
+    }
+    if (!rc) rc = join_recon(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &g);
+    c->capturing = false;
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : c->fail(VW_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c->graph_exec = nullptr;
+      return c->fail(VW_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+  }
+  HC(hipGraphLaunch(c->graph_exec, c->st));
   return sync(c);
 }
 
@@ -1548,6 +1647,7 @@ VW_API int vw_set_batch_targets(void* p, int64_t viss_waves, int64_t recon_waves
     return c->fail(VW_ERR_ARG, "target must be one of 2048, 4096, 8192, 16384, 32768, 65536");
   c->vb_target = viss_waves;
   c->rb_target = recon_waves;
+  drop_graph(c); /* the captured launches carry the old shapes */
   return VW_OK;
 }
 
@@ -1618,6 +1718,7 @@ VW_API int vw_autotune(void* p, int32_t reps, int64_t* viss_waves, int64_t* reco
   c->tim = vw_timings{};
   if (viss_waves) *viss_waves = c->vb_target;
   if (recon_waves) *recon_waves = c->rb_target;
+  drop_graph(c);
   return VW_OK;
 }
 

@@ -696,11 +696,81 @@ __global__ void k_normalize_b(const double* __restrict__ acc, const uint32_t* __
   out[s * out_stride + (scatter ? scatter[i] : i)] = v;
 }
 
-/* zero fill (kept out of hipMemsetAsync so the batched step can be captured
- * in a graph) */
-__global__ void k_fill0(double* __restrict__ p, int64_t n) {
+/* zero fill of 32-bit words (kept out of hipMemsetAsync so that the batched
+ * step can be captured in a graph: memset nodes are not replay-safe here) */
+__global__ void k_fill0(uint32_t* __restrict__ p, int64_t n) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0.0;
+  if (k < n) p[k] = 0u;
+}
+
+/* ------------------------------------------------------------------ */
+/* stable LSD radix sort without memsets (graph-capturable)             */
+/* ------------------------------------------------------------------ */
+/* One pass sorts by the 8-bit digit (key >> shift) & 255 of blocks of
+ * VW_RS_BLOCK elements: k_rs_hist counts each block's digits (LDS counters),
+ * k_rs_scan turns the digit-major table [256][nblk] into exclusive offsets,
+ * k_rs_scatter places each block's elements: lane b walks the block's digits
+ * in order and moves the elements of digit b to consecutive positions, so
+ * equal digits keep their input order (stable). */
+#define VW_RS_BLOCK 1024
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift,
+                                                 int32_t* __restrict__ hist, int64_t nblk) {
+  __shared__ int32_t h[256];
+  const int64_t b = blockIdx.x;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i0 = b * VW_RS_BLOCK, i1 = i0 + VW_RS_BLOCK < n ? i0 + VW_RS_BLOCK : n;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) atomicAdd(&h[(int)((keys[i] >> shift) & 255)], 1);
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblk + b] = h[threadIdx.x];
+}
+/* exclusive scan of m counts in place, one workgroup: each lane sums a
+ * contiguous run, the run totals are scanned in LDS, the runs rewritten */
+__global__ __launch_bounds__(1024) void k_rs_scan(int32_t* __restrict__ a, int64_t m) {
+  __shared__ int32_t t[1024];
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t i0 = threadIdx.x * per, i1 = i0 + per < m ? i0 + per : m;
+  int32_t s = 0;
+  for (int64_t i = i0; i < i1; ++i) s += a[i];
+  t[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int32_t v = threadIdx.x >= (unsigned)o ? t[threadIdx.x - o] : 0;
+    __syncthreads();
+    t[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = threadIdx.x ? t[threadIdx.x - 1] : 0;
+  for (int64_t i = i0; i < i1; ++i) {
+    const int32_t c = a[i];
+    a[i] = run;
+    run += c;
+  }
+}
+template <typename K, typename V>
+__global__ __launch_bounds__(256) void k_rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin, int64_t n,
+                                                    int shift, const int32_t* __restrict__ offs, int64_t nblk,
+                                                    K* __restrict__ kout, V* __restrict__ vout) {
+  __shared__ uint8_t d[VW_RS_BLOCK];
+  const int64_t b = blockIdx.x;
+  const int64_t i0 = b * VW_RS_BLOCK;
+  const int cnt = (int)(i0 + VW_RS_BLOCK < n ? VW_RS_BLOCK : n - i0);
+  for (int j = threadIdx.x; j < cnt; j += 256) d[j] = (uint8_t)((kin[i0 + j] >> shift) & 255);
+  __syncthreads();
+  const int bin = threadIdx.x;
+  int64_t pos = offs[(int64_t)bin * nblk + b];
+  for (int j = 0; j < cnt; ++j)
+    if (d[j] == bin) {
+      kout[pos] = kin[i0 + j];
+      vout[pos] = vin[i0 + j];
+      ++pos;
+    }
+}
+template <typename T>
+__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 #endif

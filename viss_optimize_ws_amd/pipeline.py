@@ -259,6 +259,11 @@ class Pipeline(Stages):
     def run_all(self):
         self._check(self._lib.vw_run_all(self._ctx))
 
+    def run_all_graph(self):
+        """one batched step (cleared accumulators, every segment) from a
+        captured HIP graph"""
+        self._check(self._lib.vw_run_all_graph(self._ctx))
+
     def set_day(self, day):
         self._check(self._lib.vw_set_day(self._ctx, int(day)))
 
