@@ -240,7 +240,18 @@ def _batched_leg(steps, warmup, workload):
         targets = b.autotune(reps=2)  # launch shapes of both batched kernels
         b.run_all()
         Cb, Wb, Ib, _ = b.finalize()
-        sec, tim = _time_steps(b, steps, warmup)
+        sec_eager, tim = _time_steps(b, steps, warmup)
+        # the same steps from a captured graph (reset + every segment)
+        for _ in range(max(warmup, 1)):
+            b.run_all_graph()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run_all_graph()
+            assert b._lib.vw_finalize(b._ctx, None, None, None, None) == 0
+        sec_graph = (time.perf_counter() - t0) / steps
+        Cg, Wg, Ig, _ = b.finalize()
+        graph_equal = bool(np.array_equal(Cg, Cb) and np.array_equal(Ig, Ib) and np.array_equal(Wg, Wb))
+        sec = min(sec_eager, sec_graph)
     if not SERIAL:
         with Pipeline(default_config(1, precision=1, nsky=BATCH_NSKY, seg_tlen=sched, serial_stages=1, **W)) as b2:
             b2.set_sky(skies)
@@ -264,6 +275,9 @@ def _batched_leg(steps, warmup, workload):
         "value": round(terms_sky * BATCH_NSKY / sec / 1e9, 2),
         "unit": "Gterms/s",
         "ms_per_step": round(sec * 1e3, 4),
+        "ms_per_step_eager": round(sec_eager * 1e3, 4),
+        "ms_per_step_graph": round(sec_graph * 1e3, 4),
+        "graph_equals_eager": graph_equal,
         "nsky": BATCH_NSKY,
         "terms_per_step": terms_sky * BATCH_NSKY,
         "schedule_epochs": sched,
