@@ -712,7 +712,7 @@ __global__ void k_fill0(uint32_t* __restrict__ p, int64_t n) {
  * k_rs_scatter places each block's elements: lane b walks the block's digits
  * in order and moves the elements of digit b to consecutive positions, so
  * equal digits keep their input order (stable). */
-#define VW_RS_BLOCK 1024
+#define VW_RS_BLOCK 128 /* short serial walks in the scatter (one lane per digit) */
 template <typename K>
 __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift,
                                                  int32_t* __restrict__ hist, int64_t nblk) {
