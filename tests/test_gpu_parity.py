@@ -13,7 +13,7 @@ def _sky(npix, seed=3):
     return np.random.default_rng(seed).lognormal(0.0, 1.0, npix)
 
 
-# configs[0] of the workload set: Nside 32, 8 antennas, 64 epochs, 10 MHz,
+# the CPU-sized workload: Nside 32, 8 antennas, 64 epochs, 10 MHz,
 # 4 segments, blockage=1, occ_mode=2, 256^2 image
 C1 = dict(nside=32, n_ant=8, n_epochs=64, freq_hz=1e7, segs=4, img_n=256)
 

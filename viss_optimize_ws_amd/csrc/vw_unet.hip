@@ -704,12 +704,10 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
   c->N = img_n;
   c->BM = bm;
   c->fp16 = fp16;
-  const int cs[4] = {16, 32, 64, 128};
   /* (ci, co, level) of the 14 3x3 layers in forward order, then the 1x1 */
   const int spec[15][3] = {{1, 16, 0},   {16, 16, 0},  {16, 32, 1},  {32, 32, 1},  {32, 64, 2},
                            {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {128, 64, 2}, {64, 64, 2},
                            {64, 32, 1},  {32, 32, 1},  {32, 16, 0},  {16, 16, 0},  {16, 1, 0}};
-  (void)cs;
   int64_t off = 0;
   for (int k = 0; k < 15; ++k) {
     Conv L;
@@ -746,14 +744,12 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
   /* weight-gradient partials: the largest chunks x pairs x 10 of any layer */
   {
     int64_t wp = 1;
-    const int spec_l[14][3] = {{1, 16, 0},   {16, 16, 0},  {16, 32, 1},  {32, 32, 1},  {32, 64, 2},
-                               {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {128, 64, 2}, {64, 64, 2},
-                               {64, 32, 1},  {32, 32, 1},  {32, 16, 0},  {16, 16, 0}};
     for (int k = 0; k < 14; ++k) {
-      const int64_t side = img_n >> spec_l[k][2];
+      const Conv& L = c->conv[k];
+      const int64_t side = img_n >> L.lvl;
       const int64_t tiles = (int64_t)bm * ((side + 15) / 16) * ((side + 15) / 16);
       const int64_t nch = (tiles + VW_WG_TILES - 1) / VW_WG_TILES;
-      wp = std::max<int64_t>(wp, nch * spec_l[k][0] * spec_l[k][1] * 10);
+      wp = std::max<int64_t>(wp, nch * L.ci * L.co * 10);
       /* the output layer's partials: pixel chunks x 16 channels x 2 */
       wp = std::max<int64_t>(wp, ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32);
     }
