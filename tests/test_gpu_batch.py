@@ -75,6 +75,10 @@ CASES = {
     # a segment of 8,400 rows: grouping by the batched contexts' own radix sort
     # (the one-sky path sorts with rocPRIM; equal weights mean equal groups)
     "long_segment": dict(nside=16, n_ant=16, n_epochs=72, img_n=32, nsky=20, seg_tlen=[70, 2]),
+    # 70 skies: two sky groups of four 16-sky blocks (128 padded columns)
+    "two_sky_groups": dict(nside=16, n_ant=8, n_epochs=16, segs=2, img_n=32, nsky=70),
+    # the reference's lunar formation (global occlusion threshold), 1 MHz band
+    "lunar": dict(geometry=0, nside=8, freq_hz=1e6, segs=10, img_n=16, nsky=18),
 }
 
 
@@ -83,6 +87,9 @@ def test_batch_vs_one_sky_plain(case):
     kw = dict(CASES[case])
     nsky = kw.pop("nsky")
     npix = 12 * kw["nside"] ** 2
+    if kw.get("geometry") == 0:
+        with Pipeline(config(**kw)) as p:  # the lunar plan's run is long: compare its first two segments
+            kw = dict(kw, segs=0, seg_tlen=[t for _, t in p.schedule()[:2]])
     skies = _skies(nsky, npix, 7)
     with Pipeline(config(precision=1, nsky=nsky, **kw)) as g:
         g.set_sky(skies)
