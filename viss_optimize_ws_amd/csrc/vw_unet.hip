@@ -20,6 +20,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/vissws.h"
@@ -90,6 +91,94 @@ __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const f
       out[((int64_t)n * cout + ob + o) * HH + (int64_t)y * H + x] = (T)r;
     }
   }
+}
+
+/* 3x3 convolution on the FP16 matrix cores (FP16 activations, cin a
+ * multiple of 16): an implicit GEMM D[o][pixel] = sum_K Wk[o][K] X[K][pixel]
+ * with K = tap * cin + i (tap-major).  Wk is the layer's weights re-laid in
+ * FP16 as [o][tap][i] (forward) or the flipped transpose (data gradient), so
+ * one kernel serves both.  A workgroup owns 16 output channels x a 16 x 16
+ * pixel tile of one image; each wave four pixel rows (four 16-pixel column
+ * blocks of v_mfma_f32_16x16x32_f16).  Input channels pass through LDS in
+ * chunks of 32 (16 when cin is 16), channel-last, so a lane's 8 K-values (one
+ * tap, 8 consecutive channels) are one 16-byte read. */
+typedef _Float16 vw_h8 __attribute__((ext_vector_type(8)));
+typedef float vw_f4u __attribute__((ext_vector_type(4)));
+template <bool RELU>
+__global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
+                                                    const float* __restrict__ B, _Float16* __restrict__ out, int cin,
+                                                    int cout, int H) {
+  __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * 40]; /* [y][x][ci], rows of 32 (+8 pad) */
+  const int tpr = (H + 15) / 16;
+  const int ty0 = (blockIdx.x / tpr) * 16, tx0 = (blockIdx.x % tpr) * 16;
+  const int ob = blockIdx.y * 16;
+  const int n = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t HH = (int64_t)H * H;
+  const int K = 9 * cin;
+  const int cb = cin >= 32 ? 32 : 16; /* channels per LDS chunk */
+  vw_f4u acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = vw_f4u{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < cin; c0 += cb) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 324 * cb; idx += 256) {
+      const int c = idx / 324, q = idx - c * 324, yy = q / 18, xx = q - yy * 18;
+      const int gy = ty0 + yy - 1, gx = tx0 + xx - 1;
+      _Float16 v = (_Float16)0.f;
+      if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
+      tile[(yy * 18 + xx) * 40 + c] = v;
+    }
+    __syncthreads();
+    /* k-steps of 32 K-values inside this chunk: (tap, 32 channels) or, with
+     * 16-channel chunks, (tap, 16 channels) for lanes g < 2 and the next tap
+     * for g >= 2 */
+    const int steps = cb == 32 ? 9 : 5;
+    for (int ks = 0; ks < steps; ++ks) {
+      int tap, ci0;
+      if (cb == 32) { tap = ks; ci0 = 8 * g; }
+      else { tap = 2 * ks + (g >> 1); ci0 = 8 * (g & 1); }
+      const bool kon = tap < 9;
+      /* A: weights of output channel ob + r, K = tap * cin + c0 + ci0 .. +7 */
+      vw_h8 av = {};
+      if (kon && ob + r < cout) av = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
+      const int ky = kon ? tap / 3 : 0, kx = kon ? tap % 3 : 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int py = 4 * wave + j; /* tile row of this column block */
+        vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * 40 + ci0);
+        if (!kon) bv = vw_h8{};
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[j], 0, 0, 0);
+      }
+    }
+  }
+  /* D[4 g + i][r]: output channel ob + 4 g + i, pixel (4 wave + j, r) */
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int y = ty0 + 4 * wave + j, x = tx0 + r;
+    if (y >= H || x >= H) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = ob + 4 * g + i;
+      if (o >= cout) continue;
+      float v = acc[j][i] + (B ? B[o] : 0.f);
+      if (RELU) v = v > 0.f ? v : 0.f;
+      out[((int64_t)n * cout + o) * HH + (int64_t)y * H + x] = (_Float16)v;
+    }
+  }
+}
+
+/* FP16 weight layouts for the matrix-core convolution: forward [co][tap][ci],
+ * data gradient [ci][tap][co] with the taps flipped */
+__global__ void k_wlayout(const float* __restrict__ W, int ci_n, int co_n, _Float16* __restrict__ fwd,
+                          _Float16* __restrict__ bwd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)co_n * ci_n * 9) return;
+  const int64_t co = i / (ci_n * 9), rem = i - co * ci_n * 9, ci = rem / 9, tap = rem - ci * 9;
+  const _Float16 w = (_Float16)W[i];
+  fwd[(co * 9 + tap) * ci_n + ci] = w;
+  bwd[(ci * 9 + (8 - tap)) * co_n + co] = w;
 }
 
 template <typename T>
@@ -446,6 +535,7 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 struct Conv {
   int ci, co, lvl;   /* channels and level (image side img_n >> lvl) */
   int64_t w, b;      /* offsets in the parameter vector */
+  int64_t hf, hb;    /* offsets of the FP16 forward / data-gradient weight layouts */
 };
 
 struct Net {
@@ -457,6 +547,8 @@ struct Net {
   std::vector<Conv> conv; /* 14 3x3 layers in forward order, then the 1x1 output */
   int64_t nparam = 0;
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  _Float16* wh = nullptr; /* FP16 weight layouts of the 3x3 layers (matrix-core path) */
+  int64_t nwh = 0;
   int64_t adam_t = 0;
   uint8_t* mask = nullptr;
   int64_t nmask = 0;
@@ -505,10 +597,17 @@ int ualloc(Net* c, P** p, size_t bytes) {
 
 template <typename T>
 struct Ops {
+  static constexpr bool kHalf = std::is_same<T, _Float16>::value;
   static int conv3(Net* c, const Conv& L, int src, int dst, int n) {
     const int H = c->H(L.lvl);
     const int tpr = (H + 15) / 16;
     dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.co + 15) / 16), (unsigned)n);
+    if (kHalf && L.ci % 16 == 0) { /* matrix cores */
+      k_conv3_mfma<true><<<grid, 256, 0, c->st>>>((const _Float16*)c->a[src], c->wh + L.hf, c->w + L.b,
+                                                  (_Float16*)c->a[dst], L.ci, L.co, H);
+      UC(hipGetLastError());
+      return VW_OK;
+    }
     k_conv3<T, false, true><<<grid, 256, 0, c->st>>>((const T*)c->a[src], c->w + L.w, c->w + L.b, (T*)c->a[dst], L.ci,
                                                      L.co, H);
     UC(hipGetLastError());
@@ -532,6 +631,12 @@ struct Ops {
     if (dsrc >= 0) {
       const int tpr = (H + 15) / 16;
       dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.ci + 15) / 16), (unsigned)n);
+      if (kHalf && L.co % 16 == 0) { /* matrix cores, flipped transpose */
+        k_conv3_mfma<false><<<grid, 256, 0, c->st>>>((const _Float16*)c->dz, c->wh + L.hb, nullptr,
+                                                     (_Float16*)c->d[dsrc], L.co, L.ci, H);
+        UC(hipGetLastError());
+        return VW_OK;
+      }
       k_conv3<T, true, false><<<grid, 256, 0, c->st>>>((const T*)c->dz, c->w + L.w, nullptr, (T*)c->d[dsrc], L.co,
                                                        L.ci, H);
       UC(hipGetLastError());
@@ -562,6 +667,12 @@ struct Ops {
   static int forward(Net* c, int n) {
     const std::vector<Conv>& L = c->conv;
     int rc;
+    if (kHalf) /* FP16 layouts of the current weights */
+      for (int k = 0; k < 14; ++k) {
+        k_wlayout<<<nblk((int64_t)L[k].co * L[k].ci * 9), 256, 0, c->st>>>(c->w + L[k].w, L[k].ci, L[k].co,
+                                                                          c->wh + L[k].hf, c->wh + L[k].hb);
+        UC(hipGetLastError());
+      }
 #define F(x) if ((rc = (x))) return rc
     F(conv3(c, L[0], Net::X0, Net::E0A, n));
     F(conv3(c, L[1], Net::E0A, Net::E0, n));
@@ -719,6 +830,9 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
     off += (int64_t)L.co * L.ci * ks;
     L.b = off;
     off += L.co;
+    L.hf = c->nwh;
+    L.hb = c->nwh + (int64_t)L.co * L.ci * 9;
+    if (k < 14) c->nwh += 2 * (int64_t)L.co * L.ci * 9;
     c->conv.push_back(L);
   }
   c->nparam = off;
@@ -756,6 +870,7 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
     if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp))) return rc;
   }
   const int64_t HH = (int64_t)img_n * img_n;
+  if ((rc = ualloc(c, &c->wh, sizeof(_Float16) * c->nwh))) return rc;
   if ((rc = ualloc(c, &c->w, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->g, sizeof(float) * c->nparam)) ||
       (rc = ualloc(c, &c->m, sizeof(float) * c->nparam)) || (rc = ualloc(c, &c->v, sizeof(float) * c->nparam)) ||
       (rc = ualloc(c, &c->mask, (size_t)HH)) || (rc = ualloc(c, &c->part, sizeof(float) * (bm * HH / 256 + 1))))
