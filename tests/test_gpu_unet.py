@@ -22,10 +22,11 @@ def _params(seed):
     return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in P])
 
 
-@pytest.mark.parametrize("fp16,tol", [(False, 2e-5), (True, 3e-2)])
-def test_forward_and_gradient_vs_oracle(fp16, tol):
+# 16 x 16 images: FP32 and FP16 kernels at levels of 16, 8, 4 and 2 pixels;
+# 64 x 64: every 3x3 layer but the first on the matrix cores (sides 64 .. 8)
+@pytest.mark.parametrize("fp16,tol,N,n", [(False, 2e-5, 16, 3), (True, 3e-2, 16, 3), (True, 3e-2, 64, 2)])
+def test_forward_and_gradient_vs_oracle(fp16, tol, N, n):
     rng = np.random.default_rng(21)
-    n, N = 3, 16
     x = rng.normal(size=(n, N, N))
     y = rng.normal(size=(n, N, N))
     th = _params(22)

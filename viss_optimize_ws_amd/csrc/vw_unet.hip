@@ -104,6 +104,7 @@ __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const f
  * tap, 8 consecutive channels) are one 16-byte read. */
 typedef _Float16 vw_h8 __attribute__((ext_vector_type(8)));
 typedef float vw_f4u __attribute__((ext_vector_type(4)));
+typedef _Float16 vw_h2 __attribute__((ext_vector_type(2)));
 template <bool RELU>
 __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
                                                     const float* __restrict__ B, _Float16* __restrict__ out, int cin,
@@ -117,18 +118,47 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
   const int r = lane & 15, g = lane >> 4;
   const int64_t HH = (int64_t)H * H;
   const int K = 9 * cin;
-  const int cb = cin >= 32 ? 32 : 16; /* channels per LDS chunk */
+  const int cb = cin % 32 == 0 ? 32 : 16; /* channels per LDS chunk (a divisor of cin) */
   vw_f4u acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = vw_f4u{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < cin; c0 += cb) {
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 324 * cb; idx += 256) {
-      const int c = idx / 324, q = idx - c * 324, yy = q / 18, xx = q - yy * 18;
-      const int gy = ty0 + yy - 1, gx = tx0 + xx - 1;
-      _Float16 v = (_Float16)0.f;
-      if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
-      tile[(yy * 18 + xx) * 40 + c] = v;
+    if (H % 8 == 0) {
+      /* interior columns: a lane takes a channel pair and 8 pixels of one row
+       * (two aligned 16-byte loads) and writes each pixel's pair as one dword;
+       * lanes run over the pairs first, so a write phase covers consecutive
+       * dwords */
+      const int npair = cb >> 1;
+      for (int idx = threadIdx.x; idx < npair * 36; idx += 256) {
+        const int cp = idx % npair, rest = idx / npair, yy = rest >> 1, hx = rest & 1;
+        const int gy = ty0 + yy - 1, gx = tx0 + 8 * hx;
+        vw_h8 v0 = {}, v1 = {};
+        if (gy >= 0 && gy < H && gx < H) {
+          const _Float16* s0 = in + ((int64_t)n * cin + c0 + 2 * cp) * HH + (int64_t)gy * H + gx;
+          v0 = *(const vw_h8*)s0;
+          v1 = *(const vw_h8*)(s0 + HH);
+        }
+        _Float16* d = tile + (yy * 18 + 1 + 8 * hx) * 40 + 2 * cp;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) *(vw_h2*)(d + i * 40) = vw_h2{v0[i], v1[i]};
+      }
+      /* halo columns tx0 - 1 and tx0 + 16 */
+      for (int idx = threadIdx.x; idx < cb * 36; idx += 256) {
+        const int c = idx % cb, rest = idx / cb, yy = rest >> 1, side = rest & 1;
+        const int gy = ty0 + yy - 1, gx = side ? tx0 + 16 : tx0 - 1;
+        _Float16 v = (_Float16)0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
+        tile[(yy * 18 + (side ? 17 : 0)) * 40 + c] = v;
+      }
+    } else {
+      for (int idx = threadIdx.x; idx < 324 * cb; idx += 256) {
+        const int c = idx / 324, q = idx - c * 324, yy = q / 18, xx = q - yy * 18;
+        const int gy = ty0 + yy - 1, gx = tx0 + xx - 1;
+        _Float16 v = (_Float16)0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
+        tile[(yy * 18 + xx) * 40 + c] = v;
+      }
     }
     __syncthreads();
     /* k-steps of 32 K-values inside this chunk: (tap, 32 channels) or, with
@@ -332,6 +362,162 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_t(const T* __restrict__ in,
       }
     }
   }
+}
+
+/* weight and bias gradients of a 3x3 layer on the FP16 matrix cores:
+ * dW[co][ci][tap] = sum over pixels p of dz[co][p] in[ci][p + tap] is a GEMM
+ * with M = 16 output channels, N = 16 input channels, K = pixels, and one
+ * accumulator per tap.  A workgroup owns a 16 x 16 channel block and a run of
+ * tpc tiles of 16 x 32 pixels.  Per tile it stages dz (16 rows) and the input
+ * with its halo (18 rows) in LDS as 16-byte chunks of 8 consecutive pixels,
+ * chunk-major over the 16 channels (the 16 lanes of a read phase take 256
+ * consecutive bytes); the input is stored three times, shifted by -1, 0 and +1
+ * columns, so every operand is one aligned 16-byte read.  Each wave sums four
+ * tile rows, a K-step being one row of 32 pixels: A = dz of the row (lane:
+ * output channel lane % 16, chunk lane / 16), B = the shifted input rows
+ * above, at and below it; the six input rows a wave needs are each read once
+ * and applied to every (row, ky) they serve.  Partials [chunk][co][ci][10] as
+ * for k_conv3_wgrad_t.  Requires H % 8 == 0 and co_n % 16 == 0; input
+ * channels past ci_n are staged as zeros (the first layer has one). */
+#define VW_WGM_SX (3 * 18 * 4 * 16) /* input chunks [kx][18 rows][4][16 ci] */
+#define VW_WGM_SG (16 * 4 * 16)     /* gradient chunks [16 rows][4][16 co] */
+__global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __restrict__ in,
+                                                          const _Float16* __restrict__ dz, float* __restrict__ part,
+                                                          int N, int ci_n, int co_n, int H, int tpc) {
+  __shared__ vw_h8 arena[VW_WGM_SX + VW_WGM_SG];
+  static_assert(3 * 64 * 37 * 4 <= (VW_WGM_SX + VW_WGM_SG) * 16, "reduction area");
+  vw_h8* sx = arena;
+  vw_h8* sg = arena + VW_WGM_SX;
+  const int cib = blockIdx.x * 16, cob = blockIdx.y * 16;
+  const int tpy = (H + 15) / 16, tpx = (H + 31) / 32;
+  const int64_t ntiles = (int64_t)N * tpy * tpx;
+  const int64_t t0 = (int64_t)blockIdx.z * tpc;
+  const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t HH = (int64_t)H * H;
+  vw_f4u acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = vw_f4u{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int n = (int)(t / (tpy * tpx));
+    const int rr = (int)(t - (int64_t)n * tpy * tpx);
+    const int y0 = (rr / tpx) * 16, x0 = (rr % tpx) * 32;
+    __syncthreads();
+    { /* dz: lane -> (channel tid % 16, tile row tid / 16) */
+      const int c = tid & 15, ry = tid >> 4, gy = y0 + ry;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        vw_h8 v = {};
+        if (gy < H && x0 + 8 * j < H)
+          v = *(const vw_h8*)(dz + ((int64_t)n * co_n + cob + c) * HH + (int64_t)gy * H + x0 + 8 * j);
+        sg[(ry * 4 + j) * 16 + c] = v;
+      }
+    }
+    /* input: 288 (channel, halo row) items, columns x0 - 1 .. x0 + 32 */
+    for (int item = tid; item < 288; item += 256) {
+      const int c = item & 15, hr = item >> 4, gy = y0 + hr - 1;
+      const bool rowok = gy >= 0 && gy < H && cib + c < ci_n;
+      vw_h8 v[4];
+      _Float16 hl = (_Float16)0.f, hrt = (_Float16)0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = vw_h8{};
+      if (rowok) {
+        const _Float16* src = in + ((int64_t)n * ci_n + cib + c) * HH + (int64_t)gy * H + x0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (x0 + 8 * j < H) v[j] = *(const vw_h8*)(src + 8 * j);
+        if (x0 > 0) hl = src[-1];
+        if (x0 + 32 < H) hrt = src[32];
+      }
+      vw_h8 left = {}, right = {};
+      left[7] = hl;
+      right[0] = hrt;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const vw_h8 prev = j > 0 ? v[j > 0 ? j - 1 : 0] : left;
+        const vw_h8 next = j < 3 ? v[j < 3 ? j + 1 : 3] : right;
+        sx[((0 * 18 + hr) * 4 + j) * 16 + c] = __builtin_shufflevector(prev, v[j], 7, 8, 9, 10, 11, 12, 13, 14);
+        sx[((1 * 18 + hr) * 4 + j) * 16 + c] = v[j];
+        sx[((2 * 18 + hr) * 4 + j) * 16 + c] = __builtin_shufflevector(v[j], next, 1, 2, 3, 4, 5, 6, 7, 8);
+      }
+    }
+    __syncthreads();
+    vw_h8 A[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) A[q] = sg[((4 * wave + q) * 4 + g) * 16 + r];
+    if (cib == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) bsum += (float)A[q][i];
+    }
+    /* halo row 4 wave + h serves output rows 4 wave + q with ky = h - q */
+#pragma unroll
+    for (int h = 0; h < 6; ++h) {
+      const int hr = 4 * wave + h;
+      const vw_h8 B0 = sx[((0 * 18 + hr) * 4 + g) * 16 + r];
+      const vw_h8 B1 = sx[((1 * 18 + hr) * 4 + g) * 16 + r];
+      const vw_h8 B2 = sx[((2 * 18 + hr) * 4 + g) * 16 + r];
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky) {
+        const int q = h - ky;
+        if (q < 0 || q > 3) continue;
+        acc[3 * ky + 0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[q], B0, acc[3 * ky + 0], 0, 0, 0);
+        acc[3 * ky + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[q], B1, acc[3 * ky + 1], 0, 0, 0);
+        acc[3 * ky + 2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[q], B2, acc[3 * ky + 2], 0, 0, 0);
+      }
+    }
+  }
+  /* fold the four waves through the arena (waves 1..3 write, wave 0 adds) */
+  __syncthreads();
+  float* red = reinterpret_cast<float*>(arena);
+  if (wave > 0) {
+    float* d = red + ((wave - 1) * 64 + lane) * 37;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[4 * k + i] = acc[k][i];
+    d[36] = bsum;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    for (int w = 0; w < 3; ++w) {
+      const float* d = red + (w * 64 + lane) * 37;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[k][i] += d[4 * k + i];
+      bsum += d[36];
+    }
+    /* D[4 g + i][r]: output channel cob + 4 g + i, input channel cib + r */
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float* dst = part + (((int64_t)blockIdx.z * co_n + cob + 4 * g + i) * ci_n + cib + r) * 10;
+      if (cib + r < ci_n) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dst[k] = acc[k][i];
+      }
+    }
+    /* the bias of output channel cob + r: the four chunk groups of the lanes */
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (cib == 0 && g == 0) part[(((int64_t)blockIdx.z * co_n + cob + r) * ci_n) * 10 + 9] = bsum;
+  }
+}
+
+/* first stage of a long fold: sums of VW_FOLD_GRP consecutive chunks, in
+ * chunk order, [group][per] */
+#define VW_FOLD_GRP 32
+__global__ void k_wgrad_fold_grp(const float* __restrict__ part, int nchunk, int64_t per, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per) return;
+  const int c0 = blockIdx.y * VW_FOLD_GRP;
+  const int c1 = c0 + VW_FOLD_GRP < nchunk ? c0 + VW_FOLD_GRP : nchunk;
+  float s = 0.f;
+  for (int c = c0; c < c1; ++c) s += part[c * per + i];
+  out[(int64_t)blockIdx.y * per + i] = s;
 }
 
 /* chunk partials -> dW [co][ci][9] and dB [co] (the ci = 0 partials), in chunk order */
@@ -561,7 +747,8 @@ struct Net {
   void* d[NT] = {};
   void* dz = nullptr; /* masked gradient scratch, the largest tensor */
   float* part = nullptr;
-  float* wpart = nullptr; /* weight-gradient chunk partials */
+  float* wpart = nullptr;  /* weight-gradient chunk partials */
+  float* wpart2 = nullptr; /* their sums over groups of VW_FOLD_GRP chunks */
   double* loss = nullptr; /* per-step losses on the device */
   int64_t loss_cap = 0;
   void *X = nullptr, *Y = nullptr;
@@ -585,6 +772,30 @@ struct Net {
   } while (0)
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+/* weight-gradient chunking of a 3x3 layer for a batch of n: whether the
+ * matrix-core kernel runs, its tiles per chunk, and the chunk count (about
+ * 2,048 workgroups per launch) */
+struct WgPlan {
+  bool mfma;
+  int tpc;
+  int64_t nch;
+};
+inline WgPlan wg_plan(bool fp16, const Conv& L, int H, int n) {
+  WgPlan p;
+  p.mfma = fp16 && L.co % 16 == 0 && H % 8 == 0;
+  if (p.mfma) {
+    const int64_t tiles = (int64_t)n * ((H + 15) / 16) * ((H + 31) / 32);
+    const int64_t nb = (int64_t)((L.ci + 15) / 16) * (L.co / 16);
+    p.tpc = (int)std::max<int64_t>(1, tiles * nb / 2048);
+    p.nch = (tiles + p.tpc - 1) / p.tpc;
+  } else {
+    const int64_t tpr = (H + 15) / 16;
+    p.tpc = VW_WG_TILES;
+    p.nch = ((int64_t)n * tpr * tpr + VW_WG_TILES - 1) / VW_WG_TILES;
+  }
+  return p;
+}
 
 template <typename P>
 int ualloc(Net* c, P** p, size_t bytes) {
@@ -620,13 +831,25 @@ struct Ops {
     const int64_t ne = (int64_t)n * c->elems(dst);
     k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
-    const int tpr = (H + 15) / 16;
-    const int nch = (int)(((int64_t)n * tpr * tpr + VW_WG_TILES - 1) / VW_WG_TILES);
-    k_conv3_wgrad_t<T><<<dim3((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)nch), 256, 0,
-                         c->st>>>((const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
+    const WgPlan wp = wg_plan(kHalf, L, H, n);
+    const dim3 wgrid((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)wp.nch);
+    if (wp.mfma)
+      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)c->dz, c->wpart, n,
+                                                   L.ci, L.co, H, wp.tpc);
+    else
+      k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
     UC(hipGetLastError());
-    k_wgrad_fold<<<nblk((int64_t)L.co * L.ci * 10), 256, 0, c->st>>>(c->wpart, nch, L.co, L.ci, c->g + L.w,
-                                                                     c->g + L.b);
+    const int64_t per = (int64_t)L.co * L.ci * 10;
+    const float* fsrc = c->wpart;
+    int64_t fch = wp.nch;
+    if (fch > VW_FOLD_GRP) { /* two stages keep the fold parallel (same order every call) */
+      const int64_t ng = (fch + VW_FOLD_GRP - 1) / VW_FOLD_GRP;
+      k_wgrad_fold_grp<<<dim3(nblk(per), (unsigned)ng), 256, 0, c->st>>>(c->wpart, (int)fch, per, c->wpart2);
+      UC(hipGetLastError());
+      fsrc = c->wpart2;
+      fch = ng;
+    }
+    k_wgrad_fold<<<nblk(per), 256, 0, c->st>>>(fsrc, (int)fch, L.co, L.ci, c->g + L.w, c->g + L.b);
     UC(hipGetLastError());
     if (dsrc >= 0) {
       const int tpr = (H + 15) / 16;
@@ -855,19 +1078,22 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
     if ((rc = ualloc(c, &c->a[t], bytes)) || (rc = ualloc(c, &c->d[t], bytes))) return rc;
   }
   if ((rc = ualloc(c, &c->dz, (size_t)maxe * ts))) return rc;
-  /* weight-gradient partials: the largest chunks x pairs x 10 of any layer */
+  /* weight-gradient partials: the largest chunks x pairs x 10 of any layer
+   * and batch size (the chunk count is not monotone in the batch) */
   {
-    int64_t wp = 1;
+    int64_t wp = ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32; /* output layer */
+    int64_t wp2 = 1;
     for (int k = 0; k < 14; ++k) {
       const Conv& L = c->conv[k];
-      const int64_t side = img_n >> L.lvl;
-      const int64_t tiles = (int64_t)bm * ((side + 15) / 16) * ((side + 15) / 16);
-      const int64_t nch = (tiles + VW_WG_TILES - 1) / VW_WG_TILES;
-      wp = std::max<int64_t>(wp, nch * L.ci * L.co * 10);
-      /* the output layer's partials: pixel chunks x 16 channels x 2 */
-      wp = std::max<int64_t>(wp, ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32);
+      for (int n = 1; n <= bm; ++n) {
+        const int64_t nch = wg_plan(fp16, L, img_n >> L.lvl, n).nch;
+        const int64_t per = (int64_t)L.ci * L.co * 10;
+        wp = std::max<int64_t>(wp, nch * per);
+        if (nch > VW_FOLD_GRP) wp2 = std::max<int64_t>(wp2, (nch + VW_FOLD_GRP - 1) / VW_FOLD_GRP * per);
+      }
     }
-    if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp))) return rc;
+    if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp)) || (rc = ualloc(c, &c->wpart2, sizeof(float) * wp2)))
+      return rc;
   }
   const int64_t HH = (int64_t)img_n * img_n;
   if ((rc = ualloc(c, &c->wh, sizeof(_Float16) * c->nwh))) return rc;
