@@ -590,6 +590,75 @@ __global__ void k_add(const T* __restrict__ a, const T* __restrict__ b, T* __res
   if (i < n) c[i] = (T)((float)a[i] + (float)b[i]);
 }
 
+/* FP16 forms of the element-wise kernels above: a lane takes 8 consecutive
+ * outputs along a row (16-byte accesses; the output row width a multiple of
+ * 8), with the same arithmetic per element */
+typedef _Float16 vw_h4 __attribute__((ext_vector_type(4)));
+__global__ void k_relu_bwd_h8(const vw_h8* __restrict__ dy, const vw_h8* __restrict__ y, vw_h8* __restrict__ dz,
+                              int64_t n8) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const vw_h8 a = dy[i], b = y[i];
+  vw_h8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (float)b[k] > 0.f ? a[k] : (_Float16)0.f;
+  dz[i] = r;
+}
+__global__ void k_add_h8(const vw_h8* __restrict__ a, const vw_h8* __restrict__ b, vw_h8* __restrict__ c, int64_t n8) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const vw_h8 u = a[i], v = b[i];
+  vw_h8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (_Float16)((float)u[k] + (float)v[k]);
+  c[i] = r;
+}
+/* 2x2 pooling (up_bwd = 0: average) or the adjoint of upsampling (up_bwd = 1:
+ * sum) of [nc][H][H] to [nc][H/2][H/2]; (H/2) % 8 == 0 */
+template <bool SUM>
+__global__ void k_pool_h8(const _Float16* __restrict__ x, _Float16* __restrict__ y, int64_t nc, int H) {
+  const int h = H / 2, h8 = h / 8;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * h * h8) return;
+  const int64_t c = i / ((int64_t)h * h8);
+  const int r = (int)(i - c * h * h8), yy = r / h8, xx = 8 * (r - yy * h8);
+  const _Float16* s = x + c * H * H + (int64_t)(2 * yy) * H + 2 * xx;
+  const vw_h8 a0 = *(const vw_h8*)s, a1 = *(const vw_h8*)(s + 8);
+  const vw_h8 b0 = *(const vw_h8*)(s + H), b1 = *(const vw_h8*)(s + H + 8);
+  vw_h8 o;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float v0 = ((float)a0[2 * k] + (float)a0[2 * k + 1]) + ((float)b0[2 * k] + (float)b0[2 * k + 1]);
+    const float v1 = ((float)a1[2 * k] + (float)a1[2 * k + 1]) + ((float)b1[2 * k] + (float)b1[2 * k + 1]);
+    o[k] = (_Float16)(SUM ? v0 : 0.25f * v0);
+    o[4 + k] = (_Float16)(SUM ? v1 : 0.25f * v1);
+  }
+  *(vw_h8*)(y + c * h * h + (int64_t)yy * h + xx) = o;
+}
+/* upsampling (base == nullptr, scale 1) or dx = base + pool^T(dy) (scale
+ * 1/4) of [nc][h][h] to [nc][2h][2h]; (2h) % 8 == 0 */
+template <bool POOLT>
+__global__ void k_up_h8(const _Float16* __restrict__ x, const _Float16* __restrict__ base, _Float16* __restrict__ y,
+                        int64_t nc, int h) {
+  const int H = 2 * h, H8 = H / 8;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * H * H8) return;
+  const int64_t c = i / ((int64_t)H * H8);
+  const int r = (int)(i - c * H * H8), yy = r / H8, xx = 8 * (r - yy * H8);
+  const vw_h4 s = *(const vw_h4*)(x + c * h * h + (int64_t)(yy / 2) * h + xx / 2);
+  const int64_t o = c * H * H + (int64_t)yy * H + xx;
+  vw_h8 v;
+  if (POOLT) {
+    const vw_h8 b = base ? *(const vw_h8*)(base + o) : vw_h8{};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (_Float16)((base ? (float)b[k] : 0.f) + 0.25f * (float)s[k / 2]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = s[k / 2];
+  }
+  *(vw_h8*)(y + o) = v;
+}
+
 /* 1x1 output convolution plus the input: out = b + sum_c w[c] d[c] + x */
 template <typename T>
 __global__ void k_out(const T* __restrict__ d, const T* __restrict__ x, const float* __restrict__ w,
@@ -829,7 +898,11 @@ struct Ops {
   static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n) {
     const int H = c->H(L.lvl);
     const int64_t ne = (int64_t)n * c->elems(dst);
-    k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
+    if (kHalf && ne % 8 == 0)
+      k_relu_bwd_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)c->d[dst], (const vw_h8*)c->a[dst],
+                                                      (vw_h8*)c->dz, ne / 8);
+    else
+      k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
     const WgPlan wp = wg_plan(kHalf, L, H, n);
     const dim3 wgrid((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)wp.nch);
@@ -869,20 +942,32 @@ struct Ops {
   static int pool(Net* c, int s, int t, int n) {
     const int64_t nc = (int64_t)n * c->ch[s];
     const int H = c->H(c->lv[s]);
-    k_pool<T><<<nblk(nc * (H / 2) * (H / 2)), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, H);
+    if (kHalf && (H / 2) % 8 == 0)
+      k_pool_h8<false><<<nblk(nc * (H / 2) * (H / 16)), 256, 0, c->st>>>((const _Float16*)c->a[s],
+                                                                         (_Float16*)c->a[t], nc, H);
+    else
+      k_pool<T><<<nblk(nc * (H / 2) * (H / 2)), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, H);
     UC(hipGetLastError());
     return VW_OK;
   }
   static int up(Net* c, int s, int t, int n) {
     const int64_t nc = (int64_t)n * c->ch[s];
     const int h = c->H(c->lv[s]);
-    k_up<T><<<nblk(nc * 4 * h * h), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, h);
+    if (kHalf && (2 * h) % 8 == 0)
+      k_up_h8<false><<<nblk(nc * 2 * h * (2 * h / 8)), 256, 0, c->st>>>((const _Float16*)c->a[s], nullptr,
+                                                                        (_Float16*)c->a[t], nc, h);
+    else
+      k_up<T><<<nblk(nc * 4 * h * h), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, h);
     UC(hipGetLastError());
     return VW_OK;
   }
   static int add(Net* c, int x, int y, int z, int n) {
     const int64_t ne = (int64_t)n * c->elems(z);
-    k_add<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->a[x], (const T*)c->a[y], (T*)c->a[z], ne);
+    if (kHalf && ne % 8 == 0)
+      k_add_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)c->a[x], (const vw_h8*)c->a[y], (vw_h8*)c->a[z],
+                                                 ne / 8);
+    else
+      k_add<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->a[x], (const T*)c->a[y], (T*)c->a[z], ne);
     UC(hipGetLastError());
     return VW_OK;
   }
@@ -986,7 +1071,11 @@ struct Ops {
   static int up_bwd(Net* c, int s_up, int t, int n) {
     const int64_t nc = (int64_t)n * c->ch[t];
     const int h = c->H(c->lv[t]);
-    k_up_bwd<T><<<nblk(nc * h * h), 256, 0, c->st>>>((const T*)c->d[s_up], (T*)c->d[t], nc, h);
+    if (kHalf && h % 8 == 0)
+      k_pool_h8<true><<<nblk(nc * h * (h / 8)), 256, 0, c->st>>>((const _Float16*)c->d[s_up], (_Float16*)c->d[t],
+                                                                 nc, 2 * h);
+    else
+      k_up_bwd<T><<<nblk(nc * h * h), 256, 0, c->st>>>((const T*)c->d[s_up], (T*)c->d[t], nc, h);
     UC(hipGetLastError());
     return VW_OK;
   }
@@ -994,8 +1083,12 @@ struct Ops {
   static int pool_bwd(Net* c, int s, int base, int t, int n) {
     const int64_t nc = (int64_t)n * c->ch[t];
     const int H = c->H(c->lv[t]);
-    k_pool_bwd_add<T><<<nblk(nc * H * H), 256, 0, c->st>>>((const T*)c->d[s], (const T*)c->d[base], (T*)c->d[t], nc,
-                                                           H);
+    if (kHalf && H % 8 == 0)
+      k_up_h8<true><<<nblk(nc * H * (H / 8)), 256, 0, c->st>>>((const _Float16*)c->d[s], (const _Float16*)c->d[base],
+                                                               (_Float16*)c->d[t], nc, H / 2);
+    else
+      k_pool_bwd_add<T><<<nblk(nc * H * H), 256, 0, c->st>>>((const T*)c->d[s], (const T*)c->d[base], (T*)c->d[t],
+                                                             nc, H);
     UC(hipGetLastError());
     return VW_OK;
   }
