@@ -602,7 +602,7 @@ int create_batch(Ctx* c) {
     const int64_t nmax = std::max<int64_t>(c->nq_pad, 2 * c->max_nh);
     c->rshist_cap = 256 * ((nmax + VW_RS_BLOCK - 1) / VW_RS_BLOCK);
     for (int w = 0; w < 2; ++w)
-      if ((rc = dalloc(c, &c->d_rshist[w], (size_t)c->rshist_cap))) return rc;
+      if ((rc = dalloc(c, &c->d_rshist[w], (size_t)c->rshist_cap + 256))) return rc; /* + digit totals */
   }
   if ((rc = dalloc(c, &c->d_brpart, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart, (size_t)wp))) return rc;
   if (c->ngp > 0 && ((rc = dalloc(c, &c->d_brpart2, (size_t)rp)) || (rc = dalloc(c, &c->d_bwpart2, (size_t)wp))))
@@ -868,9 +868,10 @@ int enq_rsort(Ctx* c, int which, hipStream_t st, K* k0, V* v0, K* k1, V* v1, int
   for (int p = 0; p < passes; ++p, cur ^= 1) {
     k_rs_hist<K><<<(unsigned)nblk, 256, 0, st>>>(ka[cur], n, 8 * p, h, nblk);
     KC();
-    k_rs_scan<<<1, 1024, 0, st>>>(h, 256 * nblk);
+    k_rs_scan<<<256, 256, 0, st>>>(h, nblk, h + c->rshist_cap);
     KC();
-    k_rs_scatter<K, V><<<(unsigned)nblk, 256, 0, st>>>(ka[cur], va[cur], n, 8 * p, h, nblk, ka[cur ^ 1], va[cur ^ 1]);
+    k_rs_scatter<K, V><<<(unsigned)nblk, 256, 0, st>>>(ka[cur], va[cur], n, 8 * p, h, nblk, h + c->rshist_cap,
+                                                       ka[cur ^ 1], va[cur ^ 1]);
     KC();
   }
   if (cur == 0) { /* an even number of passes ends in (k0, v0) */

@@ -708,10 +708,12 @@ __global__ void k_fill0(uint32_t* __restrict__ p, int64_t n) {
 /* ------------------------------------------------------------------ */
 /* One pass sorts by the 8-bit digit (key >> shift) & 255 of blocks of
  * VW_RS_BLOCK elements: k_rs_hist counts each block's digits (LDS counters),
- * k_rs_scan turns the digit-major table [256][nblk] into exclusive offsets,
- * k_rs_scatter places each block's elements: lane b walks the block's digits
- * in order and moves the elements of digit b to consecutive positions, so
- * equal digits keep their input order (stable). */
+ * k_rs_scan turns the digit-major table [256][nblk] into exclusive offsets
+ * within each digit plus the digit totals, k_rs_scatter places each block's
+ * elements at the digit's base (the scan of the totals) plus the block's
+ * offset: lane b walks the block's digits in order and moves the elements of
+ * digit b to consecutive positions, so equal digits keep their input order
+ * (stable). */
 #define VW_RS_BLOCK 128 /* short serial walks in the scatter (one lane per digit) */
 template <typename K>
 __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift,
@@ -725,17 +727,20 @@ __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, int
   __syncthreads();
   hist[(int64_t)threadIdx.x * nblk + b] = h[threadIdx.x];
 }
-/* exclusive scan of m counts in place, one workgroup: each lane sums a
- * contiguous run, the run totals are scanned in LDS, the runs rewritten */
-__global__ __launch_bounds__(1024) void k_rs_scan(int32_t* __restrict__ a, int64_t m) {
-  __shared__ int32_t t[1024];
-  const int64_t per = (m + 1023) / 1024;
-  const int64_t i0 = threadIdx.x * per, i1 = i0 + per < m ? i0 + per : m;
+/* exclusive scan of each digit's row of the digit-major table [256][nblk]
+ * in place, one workgroup per digit (a lane takes a contiguous run of the
+ * row, the run totals are scanned in LDS); the digit's total goes to
+ * tot[digit] */
+__global__ __launch_bounds__(256) void k_rs_scan(int32_t* __restrict__ a, int64_t nblk, int32_t* __restrict__ tot) {
+  __shared__ int32_t t[256];
+  int32_t* row = a + (int64_t)blockIdx.x * nblk;
+  const int64_t per = (nblk + 255) / 256;
+  const int64_t i0 = threadIdx.x * per, i1 = i0 + per < nblk ? i0 + per : nblk;
   int32_t s = 0;
-  for (int64_t i = i0; i < i1; ++i) s += a[i];
+  for (int64_t i = i0; i < i1; ++i) s += row[i];
   t[threadIdx.x] = s;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
+  for (int o = 1; o < 256; o <<= 1) {
     const int32_t v = threadIdx.x >= (unsigned)o ? t[threadIdx.x - o] : 0;
     __syncthreads();
     t[threadIdx.x] += v;
@@ -743,23 +748,34 @@ __global__ __launch_bounds__(1024) void k_rs_scan(int32_t* __restrict__ a, int64
   }
   int32_t run = threadIdx.x ? t[threadIdx.x - 1] : 0;
   for (int64_t i = i0; i < i1; ++i) {
-    const int32_t c = a[i];
-    a[i] = run;
+    const int32_t c = row[i];
+    row[i] = run;
     run += c;
   }
+  if (threadIdx.x == 255) tot[blockIdx.x] = t[255];
 }
 template <typename K, typename V>
 __global__ __launch_bounds__(256) void k_rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin, int64_t n,
                                                     int shift, const int32_t* __restrict__ offs, int64_t nblk,
-                                                    K* __restrict__ kout, V* __restrict__ vout) {
+                                                    const int32_t* __restrict__ tot, K* __restrict__ kout,
+                                                    V* __restrict__ vout) {
   __shared__ uint8_t d[VW_RS_BLOCK];
+  __shared__ int32_t base[256];
   const int64_t b = blockIdx.x;
   const int64_t i0 = b * VW_RS_BLOCK;
   const int cnt = (int)(i0 + VW_RS_BLOCK < n ? VW_RS_BLOCK : n - i0);
   for (int j = threadIdx.x; j < cnt; j += 256) d[j] = (uint8_t)((kin[i0 + j] >> shift) & 255);
+  /* digit bases: exclusive scan of the digit totals */
+  base[threadIdx.x] = tot[threadIdx.x];
   __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int32_t v = threadIdx.x >= (unsigned)o ? base[threadIdx.x - o] : 0;
+    __syncthreads();
+    base[threadIdx.x] += v;
+    __syncthreads();
+  }
   const int bin = threadIdx.x;
-  int64_t pos = offs[(int64_t)bin * nblk + b];
+  int64_t pos = (bin ? base[bin - 1] : 0) + offs[(int64_t)bin * nblk + b];
   for (int j = 0; j < cnt; ++j)
     if (d[j] == bin) {
       kout[pos] = kin[i0 + j];
