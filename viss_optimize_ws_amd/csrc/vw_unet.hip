@@ -105,10 +105,12 @@ __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const f
 typedef _Float16 vw_h8 __attribute__((ext_vector_type(8)));
 typedef float vw_f4u __attribute__((ext_vector_type(4)));
 typedef _Float16 vw_h2 __attribute__((ext_vector_type(2)));
-template <bool RELU>
+template <bool RELU, int CB, bool PF, bool VS>
 __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
                                                     const float* __restrict__ B, _Float16* __restrict__ out, int cin,
                                                     int cout, int H) {
+  static_assert(CB == 16 || CB == 32, "channels per LDS chunk");
+  constexpr int STEPS = CB == 32 ? 9 : 5;
   __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * 40]; /* [y][x][ci], rows of 32 (+8 pad) */
   const int tpr = (H + 15) / 16;
   const int ty0 = (blockIdx.x / tpr) * 16, tx0 = (blockIdx.x % tpr) * 16;
@@ -118,13 +120,32 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
   const int r = lane & 15, g = lane >> 4;
   const int64_t HH = (int64_t)H * H;
   const int K = 9 * cin;
-  const int cb = cin % 32 == 0 ? 32 : 16; /* channels per LDS chunk (a divisor of cin) */
+  constexpr int cb = CB; /* channels per LDS chunk (a divisor of cin) */
   vw_f4u acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = vw_f4u{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < cin; c0 += cb) {
+    /* the chunk's weight operands (k-steps: (tap, 32 channels) or, with
+     * 16-channel chunks, (tap, 16 channels) for lanes g < 2 and the next tap
+     * for g >= 2); with PF all of them are issued before the staging loads so
+     * their latency overlaps it (for launches too small to hide it by
+     * occupancy: PF costs registers), without PF one per k-step.  VS stages
+     * the tile with 16-byte loads of channel pairs; where one output-channel
+     * block covers the layer (the input tile is read once) the element-wise
+     * staging measured faster: its load instructions each touch a few cache
+     * lines of one channel. */
+    vw_h8 aw[PF ? STEPS : 1];
+    if constexpr (PF) {
+#pragma unroll
+      for (int ks = 0; ks < STEPS; ++ks) {
+        const int tap = CB == 32 ? ks : 2 * ks + (g >> 1);
+        const int ci0 = CB == 32 ? 8 * g : 8 * (g & 1);
+        aw[ks] = vw_h8{};
+        if (tap < 9 && ob + r < cout) aw[ks] = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
+      }
+    }
     __syncthreads();
-    if (H % 8 == 0) {
+    if (VS && H % 8 == 0) {
       /* interior columns: a lane takes a channel pair and 8 pixels of one row
        * (two aligned 16-byte loads) and writes each pixel's pair as one dword;
        * lanes run over the pairs first, so a write phase covers consecutive
@@ -161,27 +182,33 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
       }
     }
     __syncthreads();
-    /* k-steps of 32 K-values inside this chunk: (tap, 32 channels) or, with
-     * 16-channel chunks, (tap, 16 channels) for lanes g < 2 and the next tap
-     * for g >= 2 */
-    const int steps = cb == 32 ? 9 : 5;
-    for (int ks = 0; ks < steps; ++ks) {
-      int tap, ci0;
-      if (cb == 32) { tap = ks; ci0 = 8 * g; }
-      else { tap = 2 * ks + (g >> 1); ci0 = 8 * (g & 1); }
-      const bool kon = tap < 9;
-      /* A: weights of output channel ob + r, K = tap * cin + c0 + ci0 .. +7 */
-      vw_h8 av = {};
-      if (kon && ob + r < cout) av = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
-      const int ky = kon ? tap / 3 : 0, kx = kon ? tap % 3 : 0;
+#define VW_CONV_KSTEP(av_)                                                                      \
+  do {                                                                                          \
+    const int tap = CB == 32 ? ks : 2 * ks + (g >> 1);                                          \
+    const int ci0 = CB == 32 ? 8 * g : 8 * (g & 1);                                             \
+    const bool kon = tap < 9;                                                                   \
+    const int ky = kon ? tap / 3 : 0, kx = kon ? tap % 3 : 0;                                   \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                             \
+      const int py = 4 * wave + j; /* tile row of this column block */                          \
+      vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * 40 + ci0);                  \
+      if (!kon) bv = vw_h8{};                                                                   \
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av_, bv, acc[j], 0, 0, 0);                \
+    }                                                                                           \
+  } while (0)
+    if constexpr (PF) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int py = 4 * wave + j; /* tile row of this column block */
-        vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * 40 + ci0);
-        if (!kon) bv = vw_h8{};
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[j], 0, 0, 0);
+      for (int ks = 0; ks < STEPS; ++ks) VW_CONV_KSTEP(aw[ks]);
+    } else {
+#pragma unroll 1
+      for (int ks = 0; ks < STEPS; ++ks) {
+        const int tap = CB == 32 ? ks : 2 * ks + (g >> 1);
+        const int ci0 = CB == 32 ? 8 * g : 8 * (g & 1);
+        vw_h8 av = {};
+        if (tap < 9 && ob + r < cout) av = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
+        VW_CONV_KSTEP(av);
       }
     }
+#undef VW_CONV_KSTEP
   }
   /* D[4 g + i][r]: output channel ob + 4 g + i, pixel (4 wave + j, r) */
 #pragma unroll
@@ -875,6 +902,25 @@ int ualloc(Net* c, P** p, size_t bytes) {
   return VW_OK;
 }
 
+/* matrix-core convolution launch: 32-channel LDS chunks when cin allows;
+ * weight prefetch for launches of fewer than 4,096 workgroups; vector
+ * staging with the prefetch or when several output-channel blocks share a
+ * tile (choices measured at batch 16, 256 x 256) */
+template <bool RELU, int CB>
+void conv3_mfma_cb(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
+                   int cout, int H) {
+  const bool pf = (int64_t)grid.x * grid.y * grid.z < 4096;
+  if (pf) k_conv3_mfma<RELU, CB, true, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
+  else if (grid.y >= 2) k_conv3_mfma<RELU, CB, false, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
+  else k_conv3_mfma<RELU, CB, false, false><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
+}
+template <bool RELU>
+void conv3_mfma(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
+                int cout, int H) {
+  if (cin % 32 == 0) conv3_mfma_cb<RELU, 32>(c, grid, in, Wk, B, out, cin, cout, H);
+  else conv3_mfma_cb<RELU, 16>(c, grid, in, Wk, B, out, cin, cout, H);
+}
+
 template <typename T>
 struct Ops {
   static constexpr bool kHalf = std::is_same<T, _Float16>::value;
@@ -883,8 +929,8 @@ struct Ops {
     const int tpr = (H + 15) / 16;
     dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.co + 15) / 16), (unsigned)n);
     if (kHalf && L.ci % 16 == 0) { /* matrix cores */
-      k_conv3_mfma<true><<<grid, 256, 0, c->st>>>((const _Float16*)c->a[src], c->wh + L.hf, c->w + L.b,
-                                                  (_Float16*)c->a[dst], L.ci, L.co, H);
+      conv3_mfma<true>(c, grid, (const _Float16*)c->a[src], c->wh + L.hf, c->w + L.b, (_Float16*)c->a[dst], L.ci,
+                       L.co, H);
       UC(hipGetLastError());
       return VW_OK;
     }
@@ -928,8 +974,8 @@ struct Ops {
       const int tpr = (H + 15) / 16;
       dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.ci + 15) / 16), (unsigned)n);
       if (kHalf && L.co % 16 == 0) { /* matrix cores, flipped transpose */
-        k_conv3_mfma<false><<<grid, 256, 0, c->st>>>((const _Float16*)c->dz, c->wh + L.hb, nullptr,
-                                                     (_Float16*)c->d[dsrc], L.co, L.ci, H);
+        conv3_mfma<false>(c, grid, (const _Float16*)c->dz, c->wh + L.hb, nullptr, (_Float16*)c->d[dsrc], L.co, L.ci,
+                          H);
         UC(hipGetLastError());
         return VW_OK;
       }
