@@ -260,7 +260,7 @@ __device__ __forceinline__ float vw_block_sum(float v, float* red) {
  * (18 x 18) in LDS, and each lane accumulates 2 output x 2 input channels x
  * 9 taps (8 LDS reads per 36 FMAs), over a quarter of the tile's rows.
  * Partials [chunk][co][ci][10] (9 weights, then the bias term from the
- * ci = 0 lanes), folded in chunk order by k_wgrad_fold. */
+ * ci = 0 lanes), folded in chunk order by k_fold_stage1/2. */
 #define VW_WG_TILES 16 /* 16 x 16 pixel tiles per chunk */
 template <typename T>
 __global__ __launch_bounds__(256) void k_conv3_wgrad_t(const T* __restrict__ in, const T* __restrict__ dz,
@@ -534,31 +534,52 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
   }
 }
 
-/* first stage of a long fold: sums of VW_FOLD_GRP consecutive chunks, in
- * chunk order, [group][per] */
+/* The weight-gradient folds of all 3x3 layers of a backward pass, two
+ * launches after the last layer: stage 1 sums groups of VW_FOLD_GRP
+ * consecutive chunk partials in chunk order, [group][per] per layer; stage 2
+ * sums a layer's groups in order into dW [co][ci][9] and dB [co] (the ci = 0
+ * partials).  The same order every call, so training is bit-reproducible. */
 #define VW_FOLD_GRP 32
-__global__ void k_wgrad_fold_grp(const float* __restrict__ part, int nchunk, int64_t per, float* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= per) return;
-  const int c0 = blockIdx.y * VW_FOLD_GRP;
-  const int c1 = c0 + VW_FOLD_GRP < nchunk ? c0 + VW_FOLD_GRP : nchunk;
-  float s = 0.f;
-  for (int c = c0; c < c1; ++c) s += part[c * per + i];
-  out[(int64_t)blockIdx.y * per + i] = s;
+struct FoldJobs {
+  int njob;
+  int64_t base1[15], base2[15]; /* thread prefix sums: per x groups, per */
+  const float* part[14];        /* [nch][per] */
+  float* grp[14];               /* [ng][per]  */
+  float *dW[14], *dB[14];
+  int64_t per[14];
+  int nch[14], ng[14], ci[14];
+};
+__device__ __forceinline__ int vw_fold_job(const int64_t* base, int njob, int64_t t) {
+  int j = 0;
+  while (j + 1 < njob && t >= base[j + 1]) ++j;
+  return j;
 }
-
-/* chunk partials -> dW [co][ci][9] and dB [co] (the ci = 0 partials), in chunk order */
-__global__ void k_wgrad_fold(const float* __restrict__ part, int nchunk, int co_n, int ci_n, float* __restrict__ dW,
-                             float* __restrict__ dB) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)co_n * ci_n * 10;
-  if (i >= per) return;
+__global__ void k_fold_stage1(FoldJobs J) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= J.base1[J.njob]) return;
+  const int j = vw_fold_job(J.base1, J.njob, t);
+  const int64_t per = J.per[j], u = t - J.base1[j];
+  const int64_t gi = u / per, i = u - gi * per;
+  const int c0 = (int)gi * VW_FOLD_GRP;
+  const int c1 = c0 + VW_FOLD_GRP < J.nch[j] ? c0 + VW_FOLD_GRP : J.nch[j];
+  const float* p = J.part[j];
   float s = 0.f;
-  for (int c = 0; c < nchunk; ++c) s += part[c * per + i];
+  for (int c = c0; c < c1; ++c) s += p[c * per + i];
+  J.grp[j][gi * per + i] = s;
+}
+__global__ void k_fold_stage2(FoldJobs J) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= J.base2[J.njob]) return;
+  const int j = vw_fold_job(J.base2, J.njob, t);
+  const int64_t per = J.per[j], i = t - J.base2[j];
+  const float* g = J.grp[j];
+  float s = 0.f;
+  for (int k = 0; k < J.ng[j]; ++k) s += g[k * per + i];
+  const int ci_n = J.ci[j];
   const int64_t pair = i / 10, k = i - pair * 10;
   const int64_t co = pair / ci_n, ci = pair - co * ci_n;
-  if (k < 9) dW[(co * ci_n + ci) * 9 + k] = s;
-  else if (ci == 0) dB[co] = s;
+  if (k < 9) J.dW[j][(co * ci_n + ci) * 9 + k] = s;
+  else if (ci == 0) J.dB[j][co] = s;
 }
 
 /* dz = dy * (y > 0) */
@@ -843,8 +864,10 @@ struct Net {
   void* d[NT] = {};
   void* dz = nullptr; /* masked gradient scratch, the largest tensor */
   float* part = nullptr;
-  float* wpart = nullptr;  /* weight-gradient chunk partials */
-  float* wpart2 = nullptr; /* their sums over groups of VW_FOLD_GRP chunks */
+  float* wpart = nullptr;  /* weight-gradient chunk partials, a region per 3x3 layer, then the 1x1 layer's */
+  float* wpart2 = nullptr; /* their sums over groups of VW_FOLD_GRP chunks, a region per 3x3 layer */
+  int64_t woff[15] = {}, w2off[14] = {};
+  FoldJobs jobs{};         /* the folds queued by the current backward pass */
   double* loss = nullptr; /* per-step losses on the device */
   int64_t loss_cap = 0;
   void *X = nullptr, *Y = nullptr;
@@ -951,25 +974,30 @@ struct Ops {
       k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
     const WgPlan wp = wg_plan(kHalf, L, H, n);
+    const int li = (int)(&L - c->conv.data()); /* layer index: its partial regions */
+    float* part = c->wpart + c->woff[li];
     const dim3 wgrid((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)wp.nch);
     if (wp.mfma)
-      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)c->dz, c->wpart, n,
+      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)c->dz, part, n,
                                                    L.ci, L.co, H, wp.tpc);
     else
-      k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz, c->wpart, n, L.ci, L.co, H);
+      k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz, part, n, L.ci, L.co, H);
     UC(hipGetLastError());
-    const int64_t per = (int64_t)L.co * L.ci * 10;
-    const float* fsrc = c->wpart;
-    int64_t fch = wp.nch;
-    if (fch > VW_FOLD_GRP) { /* two stages keep the fold parallel (same order every call) */
-      const int64_t ng = (fch + VW_FOLD_GRP - 1) / VW_FOLD_GRP;
-      k_wgrad_fold_grp<<<dim3(nblk(per), (unsigned)ng), 256, 0, c->st>>>(c->wpart, (int)fch, per, c->wpart2);
-      UC(hipGetLastError());
-      fsrc = c->wpart2;
-      fch = ng;
+    { /* queue the fold (launched once per backward pass) */
+      FoldJobs& J = c->jobs;
+      const int j = J.njob++;
+      const int64_t per = (int64_t)L.co * L.ci * 10;
+      J.part[j] = part;
+      J.grp[j] = c->wpart2 + c->w2off[li];
+      J.dW[j] = c->g + L.w;
+      J.dB[j] = c->g + L.b;
+      J.per[j] = per;
+      J.nch[j] = (int)wp.nch;
+      J.ng[j] = (int)((wp.nch + VW_FOLD_GRP - 1) / VW_FOLD_GRP);
+      J.ci[j] = L.ci;
+      J.base1[j + 1] = J.base1[j] + per * J.ng[j];
+      J.base2[j + 1] = J.base2[j] + per;
     }
-    k_wgrad_fold<<<nblk(per), 256, 0, c->st>>>(fsrc, (int)fch, L.co, L.ci, c->g + L.w, c->g + L.b);
-    UC(hipGetLastError());
     if (dsrc >= 0) {
       const int tpr = (H + 15) / 16;
       dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.ci + 15) / 16), (unsigned)n);
@@ -1073,15 +1101,18 @@ struct Ops {
     UC(hipGetLastError());
     const Conv& O = L[14];
     const int onch = (int)((ne + VW_WG_CHUNK - 1) / VW_WG_CHUNK);
+    float* opart = c->wpart + c->woff[14];
     k_out_wgrad<T><<<dim3(16, (unsigned)onch), 256, 0, c->st>>>((const T*)c->d[Net::OUT], (const T*)c->a[Net::D0],
-                                                                c->wpart, n, 16, HH);
+                                                                opart, n, 16, HH);
     UC(hipGetLastError());
-    k_out_wgrad_fold<<<1, 64, 0, c->st>>>(c->wpart, onch, 16, c->g + O.w, c->g + O.b);
+    k_out_wgrad_fold<<<1, 64, 0, c->st>>>(opart, onch, 16, c->g + O.w, c->g + O.b);
     UC(hipGetLastError());
     k_out_bwd_data<T><<<nblk(ne * 16), 256, 0, c->st>>>((const T*)c->d[Net::OUT], c->w + O.w, (T*)c->d[Net::D0], n,
                                                         16, HH);
     UC(hipGetLastError());
     int rc;
+    c->jobs.njob = 0;
+    c->jobs.base1[0] = c->jobs.base2[0] = 0;
     /* decoder level 0 */
     F(conv3_bwd(c, L[13], Net::S0, Net::D0, Net::S0, n));            /* dS0 */
     F(copy(c, Net::S0, Net::D0A, n));                                /* dD0A = dS0 */
@@ -1108,6 +1139,11 @@ struct Ops {
     F(conv3_bwd(c, L[1], Net::E0A, Net::E0, Net::E0A, n));
     F(conv3_bwd(c, L[0], Net::X0, Net::E0A, -1, n));
 #undef F
+    const FoldJobs& J = c->jobs;
+    k_fold_stage1<<<nblk(J.base1[J.njob]), 256, 0, c->st>>>(J);
+    UC(hipGetLastError());
+    k_fold_stage2<<<nblk(J.base2[J.njob]), 256, 0, c->st>>>(J);
+    UC(hipGetLastError());
     return VW_OK;
   }
   static int copy(Net* c, int s, int t, int n) {
@@ -1220,17 +1256,23 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
   /* weight-gradient partials: the largest chunks x pairs x 10 of any layer
    * and batch size (the chunk count is not monotone in the batch) */
   {
-    int64_t wp = ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32; /* output layer */
-    int64_t wp2 = 1;
+    int64_t wp = 0, wp2 = 0;
     for (int k = 0; k < 14; ++k) {
       const Conv& L = c->conv[k];
+      int64_t r = 0, r2 = 0;
       for (int n = 1; n <= bm; ++n) {
         const int64_t nch = wg_plan(fp16, L, img_n >> L.lvl, n).nch;
         const int64_t per = (int64_t)L.ci * L.co * 10;
-        wp = std::max<int64_t>(wp, nch * per);
-        if (nch > VW_FOLD_GRP) wp2 = std::max<int64_t>(wp2, (nch + VW_FOLD_GRP - 1) / VW_FOLD_GRP * per);
+        r = std::max<int64_t>(r, nch * per);
+        r2 = std::max<int64_t>(r2, (nch + VW_FOLD_GRP - 1) / VW_FOLD_GRP * per);
       }
+      c->woff[k] = wp;
+      c->w2off[k] = wp2;
+      wp += (r + 63) & ~(int64_t)63;
+      wp2 += (r2 + 63) & ~(int64_t)63;
     }
+    c->woff[14] = wp; /* the output layer: pixel chunks x 16 channels x 2 */
+    wp += ((int64_t)bm * img_n * img_n + VW_WG_CHUNK - 1) / VW_WG_CHUNK * 32;
     if ((rc = ualloc(c, &c->wpart, sizeof(float) * wp)) || (rc = ualloc(c, &c->wpart2, sizeof(float) * wp2)))
       return rc;
   }
