@@ -226,16 +226,25 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
   }
 }
 
-/* FP16 weight layouts for the matrix-core convolution: forward [co][tap][ci],
- * data gradient [ci][tap][co] with the taps flipped */
-__global__ void k_wlayout(const float* __restrict__ W, int ci_n, int co_n, _Float16* __restrict__ fwd,
-                          _Float16* __restrict__ bwd) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)co_n * ci_n * 9) return;
+/* FP16 weight layouts for the matrix-core convolution, all 3x3 layers in
+ * one launch: forward [co][tap][ci], data gradient [ci][tap][co] with the taps
+ * flipped; base = prefix sums of the layers' weight counts */
+struct WLayoutJobs {
+  int64_t base[15];
+  int64_t w[14], hf[14], hb[14];
+  int ci[14], co[14];
+};
+__global__ void k_wlayout(WLayoutJobs J, const float* __restrict__ W, _Float16* __restrict__ wh) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= J.base[14]) return;
+  int j = 0;
+  while (j < 13 && t >= J.base[j + 1]) ++j;
+  const int64_t i = t - J.base[j];
+  const int64_t ci_n = J.ci[j], co_n = J.co[j];
   const int64_t co = i / (ci_n * 9), rem = i - co * ci_n * 9, ci = rem / 9, tap = rem - ci * 9;
-  const _Float16 w = (_Float16)W[i];
-  fwd[(co * 9 + tap) * ci_n + ci] = w;
-  bwd[(ci * 9 + (8 - tap)) * co_n + co] = w;
+  const _Float16 w = (_Float16)W[J.w[j] + i];
+  wh[J.hf[j] + (co * 9 + tap) * ci_n + ci] = w;
+  wh[J.hb[j] + (ci * 9 + (8 - tap)) * co_n + co] = w;
 }
 
 template <typename T>
@@ -964,14 +973,15 @@ struct Ops {
   }
   /* backward of layer L whose input is tensor src and output tensor dst with
    * gradient d[dst]: weight gradients, and (if dsrc >= 0) d[dsrc] */
-  static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n) {
+  static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n, int gdst = -1) {
     const int H = c->H(L.lvl);
     const int64_t ne = (int64_t)n * c->elems(dst);
+    const void* gd = c->d[gdst >= 0 ? gdst : dst]; /* the output's gradient (an equal tensor's when gdst >= 0) */
     if (kHalf && ne % 8 == 0)
-      k_relu_bwd_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)c->d[dst], (const vw_h8*)c->a[dst],
-                                                      (vw_h8*)c->dz, ne / 8);
+      k_relu_bwd_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)gd, (const vw_h8*)c->a[dst], (vw_h8*)c->dz,
+                                                      ne / 8);
     else
-      k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->d[dst], (const T*)c->a[dst], (T*)c->dz, ne);
+      k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)gd, (const T*)c->a[dst], (T*)c->dz, ne);
     UC(hipGetLastError());
     const WgPlan wp = wg_plan(kHalf, L, H, n);
     const int li = (int)(&L - c->conv.data()); /* layer index: its partial regions */
@@ -1049,12 +1059,19 @@ struct Ops {
   static int forward(Net* c, int n) {
     const std::vector<Conv>& L = c->conv;
     int rc;
-    if (kHalf) /* FP16 layouts of the current weights */
+    if (kHalf) { /* FP16 layouts of the current weights */
+      WLayoutJobs J{};
       for (int k = 0; k < 14; ++k) {
-        k_wlayout<<<nblk((int64_t)L[k].co * L[k].ci * 9), 256, 0, c->st>>>(c->w + L[k].w, L[k].ci, L[k].co,
-                                                                          c->wh + L[k].hf, c->wh + L[k].hb);
-        UC(hipGetLastError());
+        J.w[k] = L[k].w;
+        J.hf[k] = L[k].hf;
+        J.hb[k] = L[k].hb;
+        J.ci[k] = L[k].ci;
+        J.co[k] = L[k].co;
+        J.base[k + 1] = J.base[k] + (int64_t)L[k].co * L[k].ci * 9;
       }
+      k_wlayout<<<nblk(J.base[14]), 256, 0, c->st>>>(J, c->w, c->wh);
+      UC(hipGetLastError());
+    }
 #define F(x) if ((rc = (x))) return rc
     F(conv3(c, L[0], Net::X0, Net::E0A, n));
     F(conv3(c, L[1], Net::E0A, Net::E0, n));
@@ -1088,6 +1105,8 @@ struct Ops {
   }
 
   /* loss into loss[slot], gradients into c->g */
+  /* the skip additions S = DA + E pass their gradient unchanged to DA: the
+   * layer producing DA reads dS directly (no copy) */
   static int backward(Net* c, int n, int64_t slot) {
     const std::vector<Conv>& L = c->conv;
     const int64_t HH = (int64_t)c->N * c->N;
@@ -1115,16 +1134,13 @@ struct Ops {
     c->jobs.base1[0] = c->jobs.base2[0] = 0;
     /* decoder level 0 */
     F(conv3_bwd(c, L[13], Net::S0, Net::D0, Net::S0, n));            /* dS0 */
-    F(copy(c, Net::S0, Net::D0A, n));                                /* dD0A = dS0 */
-    F(conv3_bwd(c, L[12], Net::U0, Net::D0A, Net::U0, n));           /* dU0 */
+    F(conv3_bwd(c, L[12], Net::U0, Net::D0A, Net::U0, n, Net::S0));  /* dU0 (dD0A = dS0) */
     F(up_bwd(c, Net::U0, Net::D1, n));                               /* dD1 */
     F(conv3_bwd(c, L[11], Net::S1, Net::D1, Net::S1, n));
-    F(copy(c, Net::S1, Net::D1A, n));
-    F(conv3_bwd(c, L[10], Net::U1, Net::D1A, Net::U1, n));
+    F(conv3_bwd(c, L[10], Net::U1, Net::D1A, Net::U1, n, Net::S1));
     F(up_bwd(c, Net::U1, Net::D2, n));
     F(conv3_bwd(c, L[9], Net::S2, Net::D2, Net::S2, n));
-    F(copy(c, Net::S2, Net::D2A, n));
-    F(conv3_bwd(c, L[8], Net::U2, Net::D2A, Net::U2, n));
+    F(conv3_bwd(c, L[8], Net::U2, Net::D2A, Net::U2, n, Net::S2));
     F(up_bwd(c, Net::U2, Net::E3, n));                               /* dE3 */
     /* encoder, deepest first; the skip gradients dS* seed dE0..dE2 */
     F(conv3_bwd(c, L[7], Net::E3A, Net::E3, Net::E3A, n));
@@ -1144,10 +1160,6 @@ struct Ops {
     UC(hipGetLastError());
     k_fold_stage2<<<nblk(J.base2[J.njob]), 256, 0, c->st>>>(J);
     UC(hipGetLastError());
-    return VW_OK;
-  }
-  static int copy(Net* c, int s, int t, int n) {
-    UC(hipMemcpyAsync(c->d[t], c->d[s], (size_t)(n * c->elems(s)) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     return VW_OK;
   }
   static int up_bwd(Net* c, int s_up, int t, int n) {
