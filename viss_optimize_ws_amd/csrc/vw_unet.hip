@@ -997,6 +997,7 @@ struct Ops {
     UC(hipGetLastError());
     { /* queue the fold (launched once per backward pass) */
       FoldJobs& J = c->jobs;
+      if (J.njob >= 14) return c->fail(VW_ERR_RANGE, "internal: more than 14 weight-gradient folds in one pass");
       const int j = J.njob++;
       const int64_t per = (int64_t)L.co * L.ci * 10;
       J.part[j] = part;
