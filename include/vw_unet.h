@@ -25,7 +25,10 @@ extern "C" {
 
 /* img_n: image side, a multiple of 8 (three 2x2 poolings); batch_max:
  * largest batch of any call; act_fp16: 1 stores activations in FP16;
- * seed: He-normal initialisation (mt19937_64). */
+ * seed: He-normal initialisation (mt19937_64).  Device memory: activations
+ * and their gradients for batch_max images (0.88 GB at 256 x 256, batch 16,
+ * FP16), weight-gradient partials (0.12 GB there), the training set of
+ * vw_unet_set_data. */
 int vw_unet_create(int32_t img_n, int32_t batch_max, int32_t act_fp16, uint64_t seed, void** ctx_out);
 void vw_unet_destroy(void* ctx);
 const char* vw_unet_last_error(void* ctx);
