@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
   using Stage = VwStage<4, NB>;
   __shared__ __attribute__((aligned(16))) __bf16 bsh[2][Stage::ROWS * VW_LROW];
   __shared__ double2 grec[2][32];                      /* (ugu, vgu)      */
-  __shared__ int32_t gq[2][32];                        /* group index     */
+  __shared__ __attribute__((aligned(16))) int32_t gq[2][32]; /* group index */
   __shared__ double gfr[2][320];                       /* frames, 10 per position */
   __shared__ __attribute__((aligned(16))) uint8_t gcl[2][32]; /* classes */
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
@@ -593,29 +593,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
     /* positions of the lane: 8 g .. 8 g + 7 of the k-step */
     const int64_t jk = qa + 32 * k + g * 8;
     uint64_t cw = *(const uint64_t*)(&gcl[buf][g * 8]);
-    bool vis8 = false;
+    const vw_u32x4 q03 = *(const vw_u32x4*)(&gq[buf][g * 8]), q47 = *(const vw_u32x4*)(&gq[buf][g * 8 + 4]);
+    bool vis8 = false, all8 = jk + 8 <= qe;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) vis8 |= jk + i < qe && ((cw >> (8 * i)) & 0xffu) != VW_CLS_HIDDEN;
-    /* a k-step whose groups are all hidden from this tile adds nothing */
+    for (int i = 0; i < 8; ++i) {
+      const int cls = (int)((cw >> (8 * i)) & 0xffu);
+      const int64_t q = (int32_t)(i < 4 ? q03[i & 3] : q47[i & 3]);
+      vis8 |= jk + i < qe && cls != VW_CLS_HIDDEN;
+      all8 = all8 && cls == VW_CLS_VISIBLE && q >= q0;
+    }
+    /* a k-step whose groups are all hidden from this tile adds nothing; one
+     * whose groups are all in range, visible and of nonzero key (every lane)
+     * needs no per-term mask */
     if (wave_on && __ballot(vis8) != 0) {
       float cv[8], sv[8];
+      if (__ballot(!all8) == 0) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int64_t q = gq[buf][g * 8 + i];
-        const int cls = (int)((cw >> (8 * i)) & 0xffu);
-        bool on = jk + i < qe && q >= q0 && cls != VW_CLS_HIDDEN;
-        if (on && cls == VW_CLS_MIXED) {
-          const double* d = &gfr[buf][(g * 8 + i) * 10];
-          on = vw_visible(pl, pm, pn, d[0], d[1], d[2], d[3], d[4]) &&
-               vw_visible(pl, pm, pn, d[5], d[6], d[7], d[8], d[9]);
+        for (int i = 0; i < 8; ++i) {
+          const double2 u = grec[buf][g * 8 + i];
+          const double x = vw_recon_phase(u.x, u.y, lp, mp);
+          const float f = (float)__builtin_amdgcn_fract(x);
+          cv[i] = __builtin_amdgcn_cosf(f);
+          sv[i] = -__builtin_amdgcn_sinf(f);
         }
-        const double2 u = grec[buf][g * 8 + i];
-        const double x = vw_recon_phase(u.x, u.y, lp, mp);
-        const float f = (float)__builtin_amdgcn_fract(x);
-        const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
-        cv[i] = on ? cs : 0.0f;
-        sv[i] = on ? -sn : 0.0f;
-        cnt += on ? 1u : 0u;
+        cnt += 8u;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int64_t q = (int32_t)(i < 4 ? q03[i & 3] : q47[i & 3]);
+          const int cls = (int)((cw >> (8 * i)) & 0xffu);
+          bool on = jk + i < qe && q >= q0 && cls != VW_CLS_HIDDEN;
+          if (on && cls == VW_CLS_MIXED) {
+            const double* d = &gfr[buf][(g * 8 + i) * 10];
+            on = vw_visible(pl, pm, pn, d[0], d[1], d[2], d[3], d[4]) &&
+                 vw_visible(pl, pm, pn, d[5], d[6], d[7], d[8], d[9]);
+          }
+          const double2 u = grec[buf][g * 8 + i];
+          const double x = vw_recon_phase(u.x, u.y, lp, mp);
+          const float f = (float)__builtin_amdgcn_fract(x);
+          const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+          cv[i] = on ? cs : 0.0f;
+          sv[i] = on ? -sn : 0.0f;
+          cnt += on ? 1u : 0u;
+        }
       }
       vw_bf16x8 ach, acl, ash, asl;
       vw_split8(cv, ach, acl);
