@@ -903,7 +903,7 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 /* weight-gradient chunking of a 3x3 layer for a batch of n: whether the
  * matrix-core kernel runs, its tiles per chunk, and the chunk count (about
- * 512 workgroups per launch, one round at two per CU: fewer chunks also mean
+ * 512 workgroups per launch, all resident at once at two per CU: fewer chunks also mean
  * fewer partials to write and fold; measured against 2,048, 1,024 and 256 at
  * batch 16, 256 x 256) */
 struct WgPlan {
