@@ -436,50 +436,63 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = vw_f4u{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.f;
+  /* a tile's rows are loaded into registers one tile ahead (issued before
+   * the products of the current tile) and written to LDS at the next step:
+   * dz as lane -> (channel tid % 16, tile row tid / 16); the input as 288
+   * (channel, halo row) items, columns x0 - 1 .. x0 + 32, item tid and, for
+   * tid < 32, item tid + 256 */
+  vw_h8 gvr[4], xva[4], xvb[4];
+  _Float16 hla = (_Float16)0.f, hra = (_Float16)0.f, hlb = (_Float16)0.f, hrb = (_Float16)0.f;
+#define VW_WGM_LDX(item_, xv_, hl_, hr_)                                                  \
+  do {                                                                                    \
+    const int c_ = (item_) & 15, gy_ = y0_ + ((item_) >> 4) - 1;                          \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) xv_[j] = vw_h8{};                       \
+    hl_ = (_Float16)0.f;                                                                  \
+    hr_ = (_Float16)0.f;                                                                  \
+    if (gy_ >= 0 && gy_ < H && cib + c_ < ci_n) {                                         \
+      const _Float16* src_ = in + ((int64_t)n_ * ci_n + cib + c_) * HH + (int64_t)gy_ * H + x0_; \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j) if (x0_ + 8 * j < H) xv_[j] = *(const vw_h8*)(src_ + 8 * j); \
+      if (x0_ > 0) hl_ = src_[-1];                                                        \
+      if (x0_ + 32 < H) hr_ = src_[32];                                                   \
+    }                                                                                     \
+  } while (0)
+#define VW_WGM_LOAD(t_)                                                                   \
+  do {                                                                                    \
+    const int n_ = (int)((t_) / (tpy * tpx));                                             \
+    const int rr_ = (int)((t_) - (int64_t)n_ * tpy * tpx);                                \
+    const int y0_ = (rr_ / tpx) * 16, x0_ = (rr_ % tpx) * 32;                             \
+    const int gyd_ = y0_ + (tid >> 4);                                                    \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                       \
+      gvr[j] = vw_h8{};                                                                   \
+      if (gyd_ < H && x0_ + 8 * j < H)                                                    \
+        gvr[j] = *(const vw_h8*)(dz + ((int64_t)n_ * co_n + cob + (tid & 15)) * HH + (int64_t)gyd_ * H + x0_ + 8 * j); \
+    }                                                                                     \
+    VW_WGM_LDX(tid, xva, hla, hra);                                                       \
+    if (tid < 32) VW_WGM_LDX(tid + 256, xvb, hlb, hrb);                                   \
+  } while (0)
+#define VW_WGM_STX(item_, xv_, hl_, hr_)                                                  \
+  do {                                                                                    \
+    const int c_ = (item_) & 15, h_ = (item_) >> 4;                                       \
+    vw_h8 left_ = {}, right_ = {};                                                        \
+    left_[7] = hl_;                                                                       \
+    right_[0] = hr_;                                                                      \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                       \
+      const vw_h8 prev_ = j > 0 ? xv_[j > 0 ? j - 1 : 0] : left_;                         \
+      const vw_h8 next_ = j < 3 ? xv_[j < 3 ? j + 1 : 3] : right_;                        \
+      sx[((0 * 18 + h_) * 4 + j) * 16 + c_] = __builtin_shufflevector(prev_, xv_[j], 7, 8, 9, 10, 11, 12, 13, 14); \
+      sx[((1 * 18 + h_) * 4 + j) * 16 + c_] = xv_[j];                                     \
+      sx[((2 * 18 + h_) * 4 + j) * 16 + c_] = __builtin_shufflevector(xv_[j], next_, 1, 2, 3, 4, 5, 6, 7, 8); \
+    }                                                                                     \
+  } while (0)
+  if (t0 < t1) VW_WGM_LOAD(t0);
   for (int64_t t = t0; t < t1; ++t) {
-    const int n = (int)(t / (tpy * tpx));
-    const int rr = (int)(t - (int64_t)n * tpy * tpx);
-    const int y0 = (rr / tpx) * 16, x0 = (rr % tpx) * 32;
+    __syncthreads(); /* every wave has finished reading the previous tile */
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sg[((tid >> 4) * 4 + j) * 16 + (tid & 15)] = gvr[j];
+    VW_WGM_STX(tid, xva, hla, hra);
+    if (tid < 32) VW_WGM_STX(tid + 256, xvb, hlb, hrb);
     __syncthreads();
-    { /* dz: lane -> (channel tid % 16, tile row tid / 16) */
-      const int c = tid & 15, ry = tid >> 4, gy = y0 + ry;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        vw_h8 v = {};
-        if (gy < H && x0 + 8 * j < H)
-          v = *(const vw_h8*)(dz + ((int64_t)n * co_n + cob + c) * HH + (int64_t)gy * H + x0 + 8 * j);
-        sg[(ry * 4 + j) * 16 + c] = v;
-      }
-    }
-    /* input: 288 (channel, halo row) items, columns x0 - 1 .. x0 + 32 */
-    for (int item = tid; item < 288; item += 256) {
-      const int c = item & 15, hr = item >> 4, gy = y0 + hr - 1;
-      const bool rowok = gy >= 0 && gy < H && cib + c < ci_n;
-      vw_h8 v[4];
-      _Float16 hl = (_Float16)0.f, hrt = (_Float16)0.f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = vw_h8{};
-      if (rowok) {
-        const _Float16* src = in + ((int64_t)n * ci_n + cib + c) * HH + (int64_t)gy * H + x0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (x0 + 8 * j < H) v[j] = *(const vw_h8*)(src + 8 * j);
-        if (x0 > 0) hl = src[-1];
-        if (x0 + 32 < H) hrt = src[32];
-      }
-      vw_h8 left = {}, right = {};
-      left[7] = hl;
-      right[0] = hrt;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const vw_h8 prev = j > 0 ? v[j > 0 ? j - 1 : 0] : left;
-        const vw_h8 next = j < 3 ? v[j < 3 ? j + 1 : 3] : right;
-        sx[((0 * 18 + hr) * 4 + j) * 16 + c] = __builtin_shufflevector(prev, v[j], 7, 8, 9, 10, 11, 12, 13, 14);
-        sx[((1 * 18 + hr) * 4 + j) * 16 + c] = v[j];
-        sx[((2 * 18 + hr) * 4 + j) * 16 + c] = __builtin_shufflevector(v[j], next, 1, 2, 3, 4, 5, 6, 7, 8);
-      }
-    }
-    __syncthreads();
+    if (t + 1 < t1) VW_WGM_LOAD(t + 1);
     vw_h8 A[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) A[q] = sg[((4 * wave + q) * 4 + g) * 16 + r];
@@ -506,6 +519,9 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
       }
     }
   }
+#undef VW_WGM_LDX
+#undef VW_WGM_LOAD
+#undef VW_WGM_STX
   /* fold the four waves through the arena (waves 1..3 write, wave 0 adds) */
   __syncthreads();
   float* red = reinterpret_cast<float*>(arena);
