@@ -43,7 +43,7 @@ thread_local std::string g_unet_err;
 template <typename T, bool TR, bool RELU>
 __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const float* __restrict__ W,
                                                const float* __restrict__ B, T* __restrict__ out, int cin, int cout,
-                                               int H) {
+                                               int H, const T* __restrict__ mask) {
   __shared__ float tile[16][18][18];
   const int tpr = (H + 15) / 16;
   const int ty0 = (blockIdx.x / tpr) * 16, tx0 = (blockIdx.x % tpr) * 16;
@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const f
     if (ob + o < cout) {
       float r = acc[o] + (B ? B[ob + o] : 0.f);
       if (RELU) r = r > 0.f ? r : 0.f;
-      out[((int64_t)n * cout + ob + o) * HH + (int64_t)y * H + x] = (T)r;
+      const int64_t oi = ((int64_t)n * cout + ob + o) * HH + (int64_t)y * H + x;
+      if (mask && !((float)mask[oi] > 0.f)) r = 0.f;
+      out[oi] = (T)r;
     }
   }
 }
@@ -108,7 +110,7 @@ typedef _Float16 vw_h2 __attribute__((ext_vector_type(2)));
 template <bool RELU, int CB, bool PF, bool VS>
 __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
                                                     const float* __restrict__ B, _Float16* __restrict__ out, int cin,
-                                                    int cout, int H) {
+                                                    int cout, int H, const _Float16* __restrict__ mask) {
   static_assert(CB == 16 || CB == 32, "channels per LDS chunk");
   constexpr int STEPS = CB == 32 ? 9 : 5;
   __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * 40]; /* [y][x][ci], rows of 32 (+8 pad) */
@@ -221,7 +223,9 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
       if (o >= cout) continue;
       float v = acc[j][i] + (B ? B[o] : 0.f);
       if (RELU) v = v > 0.f ? v : 0.f;
-      out[((int64_t)n * cout + o) * HH + (int64_t)y * H + x] = (_Float16)v;
+      const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
+      if (mask && !((float)mask[oi] > 0.f)) v = 0.f;
+      out[oi] = (_Float16)v;
     }
   }
 }
@@ -628,14 +632,15 @@ __global__ void k_pool(const T* __restrict__ x, T* __restrict__ y, int64_t nc, i
 /* dx = base + pool^T(dy): each fine pixel gets a quarter of its coarse gradient */
 template <typename T>
 __global__ void k_pool_bwd_add(const T* __restrict__ dy, const T* __restrict__ base, T* __restrict__ dx, int64_t nc,
-                               int H) {
+                               int H, const T* __restrict__ mask) {
   const int h = H / 2;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc * H * H) return;
   const int64_t c = i / ((int64_t)H * H);
   const int r = (int)(i - c * H * H), y = r / H, x = r - y * H;
   const float g = 0.25f * (float)dy[c * h * h + (int64_t)(y / 2) * h + x / 2];
-  dx[i] = (T)((base ? (float)base[i] : 0.f) + g);
+  const float v = (base ? (float)base[i] : 0.f) + g;
+  dx[i] = (T)(mask && !((float)mask[i] > 0.f) ? 0.f : v);
 }
 /* nearest-neighbour upsampling of [nc][h][h] to [nc][2h][2h] */
 template <typename T>
@@ -648,14 +653,15 @@ __global__ void k_up(const T* __restrict__ x, T* __restrict__ y, int64_t nc, int
   y[i] = x[c * h * h + (int64_t)(yy / 2) * h + xx / 2];
 }
 template <typename T>
-__global__ void k_up_bwd(const T* __restrict__ dy, T* __restrict__ dx, int64_t nc, int h) {
+__global__ void k_up_bwd(const T* __restrict__ dy, T* __restrict__ dx, int64_t nc, int h, const T* __restrict__ mask) {
   const int H = 2 * h;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc * h * h) return;
   const int64_t c = i / ((int64_t)h * h);
   const int r = (int)(i - c * h * h), yy = r / h, xx = r - yy * h;
   const T* s = dy + c * H * H + (int64_t)(2 * yy) * H + 2 * xx;
-  dx[i] = (T)(((float)s[0] + (float)s[1]) + ((float)s[H] + (float)s[H + 1]));
+  const float v = ((float)s[0] + (float)s[1]) + ((float)s[H] + (float)s[H + 1]);
+  dx[i] = (T)(mask && !((float)mask[i] > 0.f) ? 0.f : v);
 }
 template <typename T>
 __global__ void k_add(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ c, int64_t n) {
@@ -689,7 +695,8 @@ __global__ void k_add_h8(const vw_h8* __restrict__ a, const vw_h8* __restrict__ 
 /* 2x2 pooling (up_bwd = 0: average) or the adjoint of upsampling (up_bwd = 1:
  * sum) of [nc][H][H] to [nc][H/2][H/2]; (H/2) % 8 == 0 */
 template <bool SUM>
-__global__ void k_pool_h8(const _Float16* __restrict__ x, _Float16* __restrict__ y, int64_t nc, int H) {
+__global__ void k_pool_h8(const _Float16* __restrict__ x, _Float16* __restrict__ y, int64_t nc, int H,
+                          const _Float16* __restrict__ mask) {
   const int h = H / 2, h8 = h / 8;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc * h * h8) return;
@@ -706,13 +713,20 @@ __global__ void k_pool_h8(const _Float16* __restrict__ x, _Float16* __restrict__
     o[k] = (_Float16)(SUM ? v0 : 0.25f * v0);
     o[4 + k] = (_Float16)(SUM ? v1 : 0.25f * v1);
   }
-  *(vw_h8*)(y + c * h * h + (int64_t)yy * h + xx) = o;
+  const int64_t oi = c * h * h + (int64_t)yy * h + xx;
+  if (mask) {
+    const vw_h8 m = *(const vw_h8*)(mask + oi);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (!((float)m[k] > 0.f)) o[k] = (_Float16)0.f;
+  }
+  *(vw_h8*)(y + oi) = o;
 }
 /* upsampling (base == nullptr, scale 1) or dx = base + pool^T(dy) (scale
  * 1/4) of [nc][h][h] to [nc][2h][2h]; (2h) % 8 == 0 */
 template <bool POOLT>
 __global__ void k_up_h8(const _Float16* __restrict__ x, const _Float16* __restrict__ base, _Float16* __restrict__ y,
-                        int64_t nc, int h) {
+                        int64_t nc, int h, const _Float16* __restrict__ mask) {
   const int H = 2 * h, H8 = H / 8;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc * H * H8) return;
@@ -728,6 +742,12 @@ __global__ void k_up_h8(const _Float16* __restrict__ x, const _Float16* __restri
   } else {
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = s[k / 2];
+  }
+  if (mask) {
+    const vw_h8 m = *(const vw_h8*)(mask + o);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (!((float)m[k] > 0.f)) v[k] = (_Float16)0.f;
   }
   *(vw_h8*)(y + o) = v;
 }
@@ -782,11 +802,12 @@ __global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ part
 /* output layer backward: dd[c] = w[c] dout, and dw[c] / db by one workgroup per c */
 template <typename T>
 __global__ void k_out_bwd_data(const T* __restrict__ dout, const float* __restrict__ w, T* __restrict__ dd, int N,
-                               int C, int64_t HH) {
+                               int C, int64_t HH, const T* __restrict__ mask) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)N * C * HH) return;
   const int64_t n = i / (C * HH), r = i - n * C * HH, c = r / HH, p = r - c * HH;
-  dd[i] = (T)(w[c] * (float)dout[n * HH + p]);
+  const float v = w[c] * (float)dout[n * HH + p];
+  dd[i] = (T)(mask && !((float)mask[i] > 0.f) ? 0.f : v);
 }
 /* one workgroup per (channel, chunk of pixels); partials [chunk][C][2] */
 template <typename T>
@@ -958,17 +979,18 @@ int ualloc(Net* c, P** p, size_t bytes) {
  * tile (choices measured at batch 16, 256 x 256) */
 template <bool RELU, int CB>
 void conv3_mfma_cb(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
-                   int cout, int H) {
+                   int cout, int H, const _Float16* mask) {
   const bool pf = (int64_t)grid.x * grid.y * grid.z < 4096;
-  if (pf) k_conv3_mfma<RELU, CB, true, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
-  else if (grid.y >= 2) k_conv3_mfma<RELU, CB, false, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
-  else k_conv3_mfma<RELU, CB, false, false><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H);
+  if (pf) k_conv3_mfma<RELU, CB, true, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else if (grid.y >= 2)
+    k_conv3_mfma<RELU, CB, false, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else k_conv3_mfma<RELU, CB, false, false><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
 }
 template <bool RELU>
 void conv3_mfma(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
-                int cout, int H) {
-  if (cin % 32 == 0) conv3_mfma_cb<RELU, 32>(c, grid, in, Wk, B, out, cin, cout, H);
-  else conv3_mfma_cb<RELU, 16>(c, grid, in, Wk, B, out, cin, cout, H);
+                int cout, int H, const _Float16* mask = nullptr) {
+  if (cin % 32 == 0) conv3_mfma_cb<RELU, 32>(c, grid, in, Wk, B, out, cin, cout, H, mask);
+  else conv3_mfma_cb<RELU, 16>(c, grid, in, Wk, B, out, cin, cout, H, mask);
 }
 
 template <typename T>
@@ -985,31 +1007,42 @@ struct Ops {
       return VW_OK;
     }
     k_conv3<T, false, true><<<grid, 256, 0, c->st>>>((const T*)c->a[src], c->w + L.w, c->w + L.b, (T*)c->a[dst], L.ci,
-                                                     L.co, H);
+                                                     L.co, H, nullptr);
     UC(hipGetLastError());
     return VW_OK;
   }
-  /* backward of layer L whose input is tensor src and output tensor dst with
-   * gradient d[dst]: weight gradients, and (if dsrc >= 0) d[dsrc] */
-  static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n, int gdst = -1) {
+  /* backward of layer L whose input is tensor src and output (a ReLU output)
+   * tensor dst.  The masked output gradient dz = d[dst] (y > 0) is d[dst]
+   * itself, masked by the kernel that produced it, when dst's gradient has
+   * no other use; with gdst >= 0 it is formed here from d[gdst], the equal
+   * gradient of a skip sum that is used elsewhere too.  Writes the weight
+   * gradients and, if dsrc >= 0, d[dsrc] (masked by a[dsrc] when mask_dsrc:
+   * dsrc is the ReLU output of the layer below and this its only use). */
+  static int conv3_bwd(Net* c, const Conv& L, int src, int dst, int dsrc, int n, int gdst = -1,
+                       bool mask_dsrc = false) {
     const int H = c->H(L.lvl);
     const int64_t ne = (int64_t)n * c->elems(dst);
-    const void* gd = c->d[gdst >= 0 ? gdst : dst]; /* the output's gradient (an equal tensor's when gdst >= 0) */
-    if (kHalf && ne % 8 == 0)
-      k_relu_bwd_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)gd, (const vw_h8*)c->a[dst], (vw_h8*)c->dz,
-                                                      ne / 8);
-    else
-      k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)gd, (const T*)c->a[dst], (T*)c->dz, ne);
-    UC(hipGetLastError());
+    const void* dzp = c->d[dst];
+    if (gdst >= 0) {
+      const void* gd = c->d[gdst];
+      if (kHalf && ne % 8 == 0)
+        k_relu_bwd_h8<<<nblk(ne / 8), 256, 0, c->st>>>((const vw_h8*)gd, (const vw_h8*)c->a[dst], (vw_h8*)c->dz,
+                                                        ne / 8);
+      else
+        k_relu_bwd<T><<<nblk(ne), 256, 0, c->st>>>((const T*)gd, (const T*)c->a[dst], (T*)c->dz, ne);
+      UC(hipGetLastError());
+      dzp = c->dz;
+    }
+    const T* dmask = mask_dsrc ? (const T*)c->a[dsrc] : nullptr;
     const WgPlan wp = wg_plan(kHalf, L, H, n);
     const int li = (int)(&L - c->conv.data()); /* layer index: its partial regions */
     float* part = c->wpart + c->woff[li];
     const dim3 wgrid((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)wp.nch);
     if (wp.mfma)
-      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)c->dz, part, n,
+      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)dzp, part, n,
                                                    L.ci, L.co, H, wp.tpc);
     else
-      k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)c->dz, part, n, L.ci, L.co, H);
+      k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)dzp, part, n, L.ci, L.co, H);
     UC(hipGetLastError());
     { /* queue the fold (launched once per backward pass) */
       FoldJobs& J = c->jobs;
@@ -1031,13 +1064,13 @@ struct Ops {
       const int tpr = (H + 15) / 16;
       dim3 grid((unsigned)(tpr * tpr), (unsigned)((L.ci + 15) / 16), (unsigned)n);
       if (kHalf && L.co % 16 == 0) { /* matrix cores, flipped transpose */
-        conv3_mfma<false>(c, grid, (const _Float16*)c->dz, c->wh + L.hb, nullptr, (_Float16*)c->d[dsrc], L.co, L.ci,
-                          H);
+        conv3_mfma<false>(c, grid, (const _Float16*)dzp, c->wh + L.hb, nullptr, (_Float16*)c->d[dsrc], L.co, L.ci,
+                          H, (const _Float16*)dmask);
         UC(hipGetLastError());
         return VW_OK;
       }
-      k_conv3<T, true, false><<<grid, 256, 0, c->st>>>((const T*)c->dz, c->w + L.w, nullptr, (T*)c->d[dsrc], L.co,
-                                                       L.ci, H);
+      k_conv3<T, true, false><<<grid, 256, 0, c->st>>>((const T*)dzp, c->w + L.w, nullptr, (T*)c->d[dsrc], L.co,
+                                                       L.ci, H, dmask);
       UC(hipGetLastError());
     }
     return VW_OK;
@@ -1047,7 +1080,7 @@ struct Ops {
     const int H = c->H(c->lv[s]);
     if (kHalf && (H / 2) % 8 == 0)
       k_pool_h8<false><<<nblk(nc * (H / 2) * (H / 16)), 256, 0, c->st>>>((const _Float16*)c->a[s],
-                                                                         (_Float16*)c->a[t], nc, H);
+                                                                         (_Float16*)c->a[t], nc, H, nullptr);
     else
       k_pool<T><<<nblk(nc * (H / 2) * (H / 2)), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, H);
     UC(hipGetLastError());
@@ -1058,7 +1091,7 @@ struct Ops {
     const int h = c->H(c->lv[s]);
     if (kHalf && (2 * h) % 8 == 0)
       k_up_h8<false><<<nblk(nc * 2 * h * (2 * h / 8)), 256, 0, c->st>>>((const _Float16*)c->a[s], nullptr,
-                                                                        (_Float16*)c->a[t], nc, h);
+                                                                        (_Float16*)c->a[t], nc, h, nullptr);
     else
       k_up<T><<<nblk(nc * 4 * h * h), 256, 0, c->st>>>((const T*)c->a[s], (T*)c->a[t], nc, h);
     UC(hipGetLastError());
@@ -1146,7 +1179,7 @@ struct Ops {
     k_out_wgrad_fold<<<1, 64, 0, c->st>>>(opart, onch, 16, c->g + O.w, c->g + O.b);
     UC(hipGetLastError());
     k_out_bwd_data<T><<<nblk(ne * 16), 256, 0, c->st>>>((const T*)c->d[Net::OUT], c->w + O.w, (T*)c->d[Net::D0], n,
-                                                        16, HH);
+                                                        16, HH, (const T*)c->a[Net::D0]); /* masked: dD0 feeds only L13 */
     UC(hipGetLastError());
     int rc;
     c->jobs.njob = 0;
@@ -1162,16 +1195,16 @@ struct Ops {
     F(conv3_bwd(c, L[8], Net::U2, Net::D2A, Net::U2, n, Net::S2));
     F(up_bwd(c, Net::U2, Net::E3, n));                               /* dE3 */
     /* encoder, deepest first; the skip gradients dS* seed dE0..dE2 */
-    F(conv3_bwd(c, L[7], Net::E3A, Net::E3, Net::E3A, n));
+    F(conv3_bwd(c, L[7], Net::E3A, Net::E3, Net::E3A, n, -1, true));
     F(conv3_bwd(c, L[6], Net::P3, Net::E3A, Net::P3, n));
     F(pool_bwd(c, Net::P3, Net::S2, Net::E2, n));                    /* dE2 = dS2 + pool^T dP3 */
-    F(conv3_bwd(c, L[5], Net::E2A, Net::E2, Net::E2A, n));
+    F(conv3_bwd(c, L[5], Net::E2A, Net::E2, Net::E2A, n, -1, true));
     F(conv3_bwd(c, L[4], Net::P2, Net::E2A, Net::P2, n));
     F(pool_bwd(c, Net::P2, Net::S1, Net::E1, n));
-    F(conv3_bwd(c, L[3], Net::E1A, Net::E1, Net::E1A, n));
+    F(conv3_bwd(c, L[3], Net::E1A, Net::E1, Net::E1A, n, -1, true));
     F(conv3_bwd(c, L[2], Net::P1, Net::E1A, Net::P1, n));
     F(pool_bwd(c, Net::P1, Net::S0, Net::E0, n));
-    F(conv3_bwd(c, L[1], Net::E0A, Net::E0, Net::E0A, n));
+    F(conv3_bwd(c, L[1], Net::E0A, Net::E0, Net::E0A, n, -1, true));
     F(conv3_bwd(c, L[0], Net::X0, Net::E0A, -1, n));
 #undef F
     const FoldJobs& J = c->jobs;
@@ -1186,9 +1219,9 @@ struct Ops {
     const int h = c->H(c->lv[t]);
     if (kHalf && h % 8 == 0)
       k_pool_h8<true><<<nblk(nc * h * (h / 8)), 256, 0, c->st>>>((const _Float16*)c->d[s_up], (_Float16*)c->d[t],
-                                                                 nc, 2 * h);
+                                                                 nc, 2 * h, (const _Float16*)c->a[t]);
     else
-      k_up_bwd<T><<<nblk(nc * h * h), 256, 0, c->st>>>((const T*)c->d[s_up], (T*)c->d[t], nc, h);
+      k_up_bwd<T><<<nblk(nc * h * h), 256, 0, c->st>>>((const T*)c->d[s_up], (T*)c->d[t], nc, h, (const T*)c->a[t]);
     UC(hipGetLastError());
     return VW_OK;
   }
@@ -1198,10 +1231,11 @@ struct Ops {
     const int H = c->H(c->lv[t]);
     if (kHalf && H % 8 == 0)
       k_up_h8<true><<<nblk(nc * H * (H / 8)), 256, 0, c->st>>>((const _Float16*)c->d[s], (const _Float16*)c->d[base],
-                                                               (_Float16*)c->d[t], nc, H / 2);
+                                                               (_Float16*)c->d[t], nc, H / 2,
+                                                               (const _Float16*)c->a[t]);
     else
       k_pool_bwd_add<T><<<nblk(nc * H * H), 256, 0, c->st>>>((const T*)c->d[s], (const T*)c->d[base], (T*)c->d[t],
-                                                             nc, H);
+                                                             nc, H, (const T*)c->a[t]);
     UC(hipGetLastError());
     return VW_OK;
   }
