@@ -21,7 +21,7 @@ Tile cones only accelerate; the fixtures hold the pure per-point predicate
 results, which equal the tile-cone form whenever the cone invariant holds
 (checked separately).
 
-Run:  python tests/golden/make_golden.py   (writes tests/golden/*.npz)
+Run:  python tests/golden/make_golden.py [case ...]   (writes tests/golden/*.npz)
 """
 import math
 import os
@@ -385,7 +385,47 @@ CASES = {
                              img_n=16, blockage=0, occ_mode=2, recon_mode=0, sky_seed=13, day=1),
     "lunar_mirror_occ0": dict(geometry="lunar8", nside=4, n_ant=8, freq_hz=1e6, orbit_seed=42, seg_tlen=[12, 12],
                               img_n=16, blockage=1, occ_mode=0, recon_mode=0, sky_seed=14, day=3),
+    # the CPU-sized workload in full: Nside 32 (48 tiles of 256 pixels, so the
+    # tile-cone classes are a mix of visible, hidden and mixed), 8 antennas,
+    # 64 epochs in 4 segments of 16, 10 MHz, 256^2 image
+    "circ_c1_mirror_occ2": dict(geometry="circular", nside=32, n_ant=8, freq_hz=1e7, orbit_seed=42,
+                                seg_tlen=[16, 16, 16, 16], img_n=256, blockage=1, occ_mode=2, recon_mode=0,
+                                sky_seed=15, day=1, compact=True),
+    # first member of the sorted run (viss_recon_kernel.cuh:603-604)
+    "circ_mirror_occ1": dict(geometry="circular", nside=8, n_ant=7, freq_hz=2e7, orbit_seed=5, seg_tlen=[7, 9],
+                             img_n=24, blockage=1, occ_mode=1, recon_mode=0, sky_seed=16, day=1),
+    # median member on the full uv grid (main_recon_online.cu form)
+    "circ_full_occ2": dict(geometry="circular", nside=8, n_ant=6, freq_hz=1e7, orbit_seed=9, seg_tlen=[9, 6],
+                           img_n=24, blockage=1, occ_mode=2, recon_mode=1, sky_seed=17, day=1),
+    # any other occ_mode value takes the four-sample rule (the else branch,
+    # viss_recon_kernel.cuh:605-612)
+    "circ_mirror_occ4": dict(geometry="circular", nside=8, n_ant=6, freq_hz=1e7, orbit_seed=21, seg_tlen=[10],
+                             img_n=16, blockage=1, occ_mode=4, recon_mode=0, sky_seed=18, day=1),
 }
+
+
+def compact(flat):
+    """large cases keep the per-row occlusion masks as a SHA-256 of their
+    packed bits (the test compares the oracle's predicate by the same
+    digest), the image inside the unit disk only and every 8th pixel
+    direction"""
+    import hashlib
+
+    out = {}
+    for k, v in flat.items():
+        if k == "lmn":
+            out["lmn_stride8"] = v[:, ::8]
+        elif k.endswith("/vis"):
+            out[k + "_sha256"] = np.frombuffer(hashlib.sha256(np.packbits(v.astype(bool))).digest(), np.uint8)
+        elif k in ("img", "Wimg"):
+            n = v.shape[0]
+            j = np.arange(n)
+            disk = ((2 * j + 1 - n)[None, :] ** 2 + (2 * j + 1 - n)[:, None] ** 2) < n * n
+            out[k + "_disk"] = v[disk]
+            out["img_n"] = np.asarray(n)
+        else:
+            out[k] = v
+    return out
 
 
 def flatten(res):
@@ -400,7 +440,13 @@ def flatten(res):
 
 
 if __name__ == "__main__":
-    for name, cfg in CASES.items():
-        res = run(cfg)
-        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **flatten(res))
+    import sys
+
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        cfg = CASES[name]
+        flat = flatten(run(cfg))
+        if cfg.get("compact"):
+            flat = compact(flat)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **flat)
         print("wrote", name)

@@ -98,6 +98,7 @@ def _seg_checks(tol, exact_groups=True):
         dict(occ_mode=0),
         dict(occ_mode=1),
         dict(occ_mode=3),
+        dict(occ_mode=4),
         dict(recon_mode=1, occ_mode=3),
         dict(blockage=0),
         dict(recon_mode=1, blockage=0),

@@ -9,6 +9,7 @@
 3. Invariants: the tile-cone classes never disagree with the per-pixel
    predicate, conjugate completion, uvw half symmetry, HEALPix geometry.
 """
+import hashlib
 import math
 import os
 
@@ -50,7 +51,10 @@ def test_oracle_matches_golden(name):
         npix = o.plan.npix
         assert o.schedule() == list(zip(np.cumsum([0] + case["seg_tlen"][:-1]).tolist(), case["seg_tlen"]))
         l, m, n, nm1 = o.lmn()
-        assert np.max(np.abs(np.stack([l, m, n]) - gold["lmn"])) < 2e-15
+        if "lmn" in gold.files:
+            assert np.max(np.abs(np.stack([l, m, n]) - gold["lmn"])) < 2e-15
+        else:
+            assert np.max(np.abs(np.stack([l, m, n])[:, ::8] - gold["lmn_stride8"])) < 2e-15
         pos = o.positions()[:, : sum(case["seg_tlen"]), :]
         assert rel(pos, gold["pos"]) < 1e-13
         assert abs(o.plan.bl_max_m - float(gold["blmax"])) <= 1e-12 * float(gold["blmax"])
@@ -62,7 +66,11 @@ def test_oracle_matches_golden(name):
             u, v, w = o.rows()
             assert rel(u, g["u"]) < 1e-12 and rel(v, g["v"]) < 1e-12 and rel(w, g["w"]) < 1e-12
             # pure predicate == ray-sphere / fixed-cone brute force, bit for bit
-            assert np.array_equal(o.predicate(0, o.nh).astype(bool), g["vis"])
+            pred = o.predicate(0, o.nh).astype(bool)
+            if "vis" in g:
+                assert np.array_equal(pred, g["vis"])
+            else:
+                assert hashlib.sha256(np.packbits(pred)).digest() == g["vis_sha256"].tobytes()
             vraw = o.viss_half()
             assert rel(vraw, g["vraw"]) < 1e-8  # phases amplify 1e-16 position differences by |u| ~ 1e6
             vh, vf = o.conj_fill()
@@ -78,9 +86,16 @@ def test_oracle_matches_golden(name):
             assert np.array_equal(srt, g["idx_sorted"])
         Cm, W, img, Wi = o.finalize()
         assert np.array_equal(W, gold["W"])
-        assert np.array_equal(Wi, gold["Wimg"])
         assert rel(Cm, gold["C"]) < 1e-8
-        assert rel(img, gold["img"]) < 1e-8
+        if "img" in gold.files:
+            assert np.array_equal(Wi, gold["Wimg"])
+            assert rel(img, gold["img"]) < 1e-8
+        else:
+            nn = int(gold["img_n"])
+            j = np.arange(nn)
+            disk = ((2 * j + 1 - nn)[None, :] ** 2 + (2 * j + 1 - nn)[:, None] ** 2) < nn * nn
+            assert np.array_equal(Wi[disk], gold["Wimg_disk"]) and not Wi[~disk].any()
+            assert rel(img[disk], gold["img_disk"]) < 1e-8 and not img[~disk].any()
 
 
 def test_mt19937_64_known_answer():
