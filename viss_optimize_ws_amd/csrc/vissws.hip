@@ -332,9 +332,10 @@ int make_plan(Ctx* c, const vw_config* cfg) {
   c->nq_pad = (2 * c->max_nh + 63) / 64 * 64;
   /* occ_mode 1 and 2 read one member per group: the representative kernel,
    * with the finite-value filter of the member sets except for the uv-mirror
-   * occ_mode 2 form (recon_seg_blockage_half_rep); occ_mode 0 and 3 average
-   * several members per output point */
-  c->generic = cfg->blockage && (cfg->occ_mode == 0 || cfg->occ_mode == 3);
+   * occ_mode 2 form (recon_seg_blockage_half_rep); occ_mode 0 averages every
+   * member and any other value the four samples s0, s0+L/3, s0+2L/3, s1-1 per
+   * output point (the else branch of viss_recon_kernel.cuh:605-612) */
+  c->generic = cfg->blockage && cfg->occ_mode != 1 && cfg->occ_mode != 2;
   c->nsky = cfg->nsky > 1 ? cfg->nsky : 1;
   c->batch = c->nsky > 1;
   if (c->batch) {

@@ -86,10 +86,11 @@ struct Ckpt {
   char magic[8];
   uint64_t fingerprint; /* run parameters that must match on resume */
   int32_t day, next_seg; /* next work item: segment next_seg of day      */
+  int32_t day_start, day_count; /* day range of the run that wrote it    */
   int64_t npix, ngp, wtotal;
   int64_t fsize[4]; /* Viss, uvw, xyza, xyzb text sizes (-1: absent)     */
 };
-const char kMagic[8] = {'V', 'W', 'C', 'K', 'P', 'T', '0', '1'};
+const char kMagic[8] = {'V', 'W', 'C', 'K', 'P', 'T', '0', '2'};
 
 struct Fnv {
   uint64_t h = 1469598103934665603ULL;
@@ -161,7 +162,7 @@ int main(int argc, char** argv) {
   int segs = to_int(get_arg(argc, argv, "--segs", "0"), 0);
   cfg.blockage = to_int(get_arg(argc, argv, "--blockage", "1"), 1);
   cfg.occ_mode = to_int(get_arg(argc, argv, "--occ_mode", "2"), 2);
-  int write_viss = to_int(get_arg(argc, argv, "--write_viss", "0"), 0);
+  int write_viss = to_int(get_arg(argc, argv, "--write_viss", "0"), 1); /* 1 on a parse error, as main_recon_online.cu:13 */
   int write_baseline_txt = to_int(get_arg(argc, argv, "--write_baseline_txt", "0"), 0);
   std::string B_mode = get_arg(argc, argv, "--B_mode", "auto");
   std::string C_mode = get_arg(argc, argv, "--C_mode", "txt");
@@ -286,6 +287,10 @@ int main(int argc, char** argv) {
   fp.v(cfg.viss_form);
   fp.v(cfg.occ_radius_km); fp.v(cfg.occ_altitude_km); fp.v(cfg.r_min_km); fp.v(cfg.r_max_km);
   fp.str(btag); fp.str(sky_kind); fp.v(sky_seed); fp.str(sky_kind == "file" ? sky_dir : std::string());
+  fp.str(sky_kind == "file" ? B_mode : std::string());
+  /* the loaded sky itself: another file (B_10M.bin vs B.txt) or an edited one
+   * in the same directory must not resume into these accumulators */
+  fp.add(B.data(), B.size() * sizeof(double));
   fp.v(write_viss); fp.v(write_baseline_txt);
   for (int k = 0; k < plan.segs; ++k) { fp.v(seg_t0[k]); fp.v(seg_tlen[k]); }
   std::vector<double> ckC, ckG;
@@ -303,6 +308,11 @@ int main(int argc, char** argv) {
     if (r > 0) {
       if (h.fingerprint != fp.h || h.npix != npix || h.ngp != plan.n_img_pts) {
         std::fprintf(stderr, "ERROR checkpoint %s was written by a run with other parameters\n", ckpt_path.c_str());
+        return 1;
+      }
+      if (h.day_start != day_start) {
+        std::fprintf(stderr, "ERROR checkpoint %s was written by a run starting at day %d, not %d\n", ckpt_path.c_str(),
+                     h.day_start, day_start);
         return 1;
       }
       if (h.day < day_start || h.day > day_start + day_count || h.next_seg < 0 || h.next_seg > plan.segs) {
@@ -335,6 +345,8 @@ int main(int argc, char** argv) {
     h.fingerprint = fp.h;
     h.day = dayid;
     h.next_seg = next_seg;
+    h.day_start = day_start;
+    h.day_count = day_count;
     h.npix = npix;
     h.ngp = plan.n_img_pts;
     h.wtotal = 0;
