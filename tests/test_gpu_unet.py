@@ -25,8 +25,14 @@ def _params(seed):
 # 16 x 16 images: FP32 and FP16 kernels at levels of 16, 8, 4 and 2 pixels;
 # 64 x 64: every 3x3 layer but the first on the matrix cores (sides 64 .. 8),
 # and with 6 images the level-0 weight gradients fold 48 chunk partials in
-# two groups
-@pytest.mark.parametrize("fp16,tol,N,n", [(False, 2e-5, 16, 3), (True, 3e-2, 16, 3), (True, 3e-2, 64, 6)])
+# two groups.  The FP16 kernels are compared with the oracle's FP16 storage
+# mode (activations, gradients and the matrix-core weights rounded to FP16
+# where the kernels store them; sums FP64 there and FP32 here), so what is
+# left is FP32 summation order plus the occasional stored value that rounds
+# to the neighbouring FP16 value: measured up to 9e-4 (output) and 4.1e-3
+# (the level-3 weight gradient at 64 x 64, sums of 384 terms) of the largest
+# entry.  FP32 activations against the FP64 oracle: 2e-5 (gradients 1e-4).
+@pytest.mark.parametrize("fp16,tol,N,n", [(False, 2e-5, 16, 3), (True, 5e-3, 16, 3), (True, 5e-3, 64, 6)])
 def test_forward_and_gradient_vs_oracle(fp16, tol, N, n):
     rng = np.random.default_rng(21)
     x = rng.normal(size=(n, N, N))
@@ -38,16 +44,17 @@ def test_forward_and_gradient_vs_oracle(fp16, tol, N, n):
         th32 = net.params().astype(np.float64)  # the FP32 parameters the kernels use
         out = net.predict(x.astype(np.float32))
         loss, g = net.loss_grad(x.astype(np.float32), y.astype(np.float32))
-    ref, _ = U.forward(th32, x.astype(np.float32))
-    lref, gref = U.loss_grad(th32, x.astype(np.float32), y.astype(np.float32))
+    ref, _ = U.forward(th32, x.astype(np.float32), fp16=fp16)
+    lref, gref = U.loss_grad(th32, x.astype(np.float32), y.astype(np.float32), fp16=fp16)
     assert np.max(np.abs(out - ref)) <= tol * np.max(np.abs(ref))
     assert abs(loss - lref) <= tol * lref
-    # per layer, relative to the layer's largest gradient
+    # per layer, relative to the layer's largest gradient (FP32: 1e-4)
+    gtol = tol if fp16 else 5 * tol
     off = 0
     for k, (ci, co, _) in enumerate(U.SPEC):
         m = co * ci * (9 if k < 14 else 1) + co
         gl, rl = g[off:off + m], gref[off:off + m]
-        assert np.max(np.abs(gl - rl)) <= 5 * tol * max(np.max(np.abs(rl)), 1e-30), (k, fp16)
+        assert np.max(np.abs(gl - rl)) <= gtol * max(np.max(np.abs(rl)), 1e-30), (k, fp16)
         off += m
 
 

@@ -93,3 +93,38 @@ def test_grid_pixel_map():
     ng = np.sqrt(1 - lg[inside] ** 2 - mg[inside] ** 2)
     ang = np.arccos(np.clip(cl * lg[inside] + cm * mg[inside] + cn * ng, -1, 1))
     assert ang.max() < 2.0 * np.sqrt(4 * np.pi / (12 * nside * nside))
+
+
+def test_fp16_storage_mode():
+    """the oracle's FP16 storage mode: every stored activation and the output
+    are FP16 values, the first layer's FP32 weights and the FP16 weights of
+    the others are the ones applied (a forward pass with those weights
+    rounded by hand agrees), results stay near the FP64 network (a few
+    percent of a layer's largest gradient at 16 x 16: FP16 rounding
+    propagated through 30 layers and their ReLU masks), and the rounding is
+    really applied"""
+    th = _rand_params(13)
+    rng = np.random.default_rng(14)
+    x = rng.normal(size=(2, 16, 16))
+    y = rng.normal(size=(2, 16, 16))
+    out16, a16 = U.forward(th, x, fp16=True)
+    for k, v in a16.items():
+        assert np.array_equal(v, v.astype(np.float16).astype(np.float64)), k
+    assert np.array_equal(out16, out16.astype(np.float16).astype(np.float64))
+    # FP16 weights for layers 1..13 only: rounding them by hand and running
+    # the FP16 mode again changes nothing, rounding layer 0 or 14 does
+    P = U.unpack(th.copy())
+    th_r = np.concatenate([np.concatenate([(U._f16(W) if 0 < k < 14 else W).ravel(), b.ravel()])
+                           for k, (W, b) in enumerate(P)])
+    assert np.array_equal(U.forward(th_r, x, fp16=True)[0], out16)
+    out64, _ = U.forward(th, x)
+    assert 0 < np.max(np.abs(out16 - out64)) <= 2e-2 * np.max(np.abs(out64))
+    l64, g64 = U.loss_grad(th, x, y)
+    l16, g16 = U.loss_grad(th, x, y, fp16=True)
+    assert abs(l16 - l64) <= 1e-2 * l64
+    off = 0
+    for k, (ci, co, _) in enumerate(U.SPEC):
+        m = co * ci * (9 if k < 14 else 1) + co
+        a, b = g16[off:off + m], g64[off:off + m]
+        assert 0 < np.max(np.abs(a - b)) <= 1e-1 * np.max(np.abs(b)), k
+        off += m
