@@ -51,6 +51,15 @@ int vw_unet_set_data(void* ctx, const float* X, const float* Y, int64_t count);
 int vw_unet_train(void* ctx, int64_t steps, int32_t batch, float lr, uint64_t seed, float* losses);
 /* Network output for n images (host in, host out). */
 int vw_unet_predict(void* ctx, const float* X, int32_t n, float* out);
+/* vw_unet_predict that allocates no device memory: the FP32 staging of the
+ * images in and out goes through a static device array of the library
+ * (16 MiB: 4,194,304 values, e.g. 64 images of 256 x 256), whose address on
+ * the device is resolved by vw_unet_create.  The largest batch,
+ * min(batch_max, n) x img_n^2 values, must fit it (VW_ERR_RANGE otherwise).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory at call time (tight serving loops, harnesses that forbid hipMalloc).
+ * Calls on one device share the array; calls that overlap there race. */
+int vw_unet_predict_noalloc(void* ctx, const float* X, int32_t n, float* out);
 /* Device time of the last vw_unet_train call (ms, HIP events). */
 int vw_unet_train_ms(void* ctx, double* ms);
 

@@ -181,6 +181,7 @@ UNET_FUNCS = {
     "unet_set_data": (C.c_int, [P, PF, PF, i64]),
     "unet_train": (C.c_int, [P, i64, i32, C.c_float, C.c_uint64, PF]),
     "unet_predict": (C.c_int, [P, PF, i32, PF]),
+    "unet_predict_noalloc": (C.c_int, [P, PF, i32, PF]),
     "unet_train_ms": (C.c_int, [P, PD]),
 }
 

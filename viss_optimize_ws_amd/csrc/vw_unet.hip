@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <type_traits>
@@ -27,6 +28,15 @@
 #include "../../include/vw_unet.h"
 
 #define VW_API extern "C" __attribute__((visibility("default")))
+
+/* Staging of vw_unet_predict_noalloc: FP32 images in and out of the device
+ * in a buffer of the library's own, so that entry point allocates nothing.
+ * 16 MiB (4,194,304 values: 64 images of 256 x 256), held by every device the
+ * library runs on until the process ends.  In host code this name is not a
+ * device address: vw_unet_create resolves the address on its device with
+ * hipGetSymbolAddress, and only that address is used. */
+#define VW_UNET_SCRATCH_FLOATS 4194304
+static __device__ __align__(256) float scratch[VW_UNET_SCRATCH_FLOATS];
 
 namespace {
 
@@ -921,6 +931,7 @@ struct Net {
   int32_t* idx = nullptr;
   int64_t idx_cap = 0;
   double train_ms = 0.0;
+  float* scratch_dev = nullptr; /* this device's address of scratch (vw_unet_predict_noalloc) */
 
   int fail(int code, const std::string& e) {
     err = e;
@@ -1239,29 +1250,31 @@ struct Ops {
     UC(hipGetLastError());
     return VW_OK;
   }
-  static int upload(Net* c, const float* host, int t, int n, bool grad = false) {
+  /* host FP32 -> tensor t of the storage type, through a staging buffer:
+   * allocated per call, or the caller's (staging, no allocation) */
+  static int upload(Net* c, const float* host, int t, int n, bool grad = false, float* staging = nullptr) {
     const int64_t ne = n * c->elems(t);
-    float* tmp = nullptr;
-    UC(hipMalloc(&tmp, sizeof(float) * ne));
+    float* tmp = staging;
+    if (!tmp) UC(hipMalloc(&tmp, sizeof(float) * ne));
     hipError_t e = hipMemcpyAsync(tmp, host, sizeof(float) * ne, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) {
       k_from_f32<T><<<nblk(ne), 256, 0, c->st>>>(tmp, (T*)(grad ? c->d[t] : c->a[t]), ne);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    (void)hipFree(tmp);
+    if (!staging) (void)hipFree(tmp);
     if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
     return VW_OK;
   }
-  static int download(Net* c, int t, int n, float* host) {
+  static int download(Net* c, int t, int n, float* host, float* staging = nullptr) {
     const int64_t ne = n * c->elems(t);
-    float* tmp = nullptr;
-    UC(hipMalloc(&tmp, sizeof(float) * ne));
+    float* tmp = staging;
+    if (!tmp) UC(hipMalloc(&tmp, sizeof(float) * ne));
     k_to_f32<T><<<nblk(ne), 256, 0, c->st>>>((const T*)c->a[t], tmp, ne);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, sizeof(float) * ne, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    (void)hipFree(tmp);
+    if (!staging) (void)hipFree(tmp);
     if (e != hipSuccess) return c->fail(VW_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
     return VW_OK;
   }
@@ -1375,6 +1388,26 @@ int build(Net* c, int img_n, int bm, bool fp16, uint64_t seed) {
 
 Net* as_net(void* p) { return (Net*)p; }
 
+/* Address of scratch on the current device, resolved once per device (with
+ * that device current) and kept in a table by device ordinal. */
+std::mutex g_scratch_mu;
+std::vector<float*> g_scratch_addr;
+
+int resolve_scratch(Net* c) {
+  int dev = 0;
+  UC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_addr[(size_t)dev]) {
+    void* a = nullptr;
+    UC(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    if (!a) return c->fail(VW_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_addr[(size_t)dev] = (float*)a;
+  }
+  c->scratch_dev = g_scratch_addr[(size_t)dev];
+  return VW_OK;
+}
+
 template <typename F>
 int dispatch(Net* c, F&& f) {
   return c->fp16 ? f(Ops<_Float16>{}) : f(Ops<float>{});
@@ -1396,6 +1429,7 @@ VW_API int vw_unet_create(int32_t img_n, int32_t batch_max, int32_t act_fp16, ui
   }
   Net* c = new Net();
   int rc = build(c, img_n, batch_max, act_fp16 != 0, seed);
+  if (!rc) rc = resolve_scratch(c);
   if (rc) {
     g_unet_err = c->err;
     vw_unet_destroy(c);
@@ -1588,6 +1622,36 @@ VW_API int vw_unet_predict(void* p, const float* X, int32_t n, float* out) {
       using O = decltype(ops);
       int r;
       if ((r = O::upload(c, X + b0 * HH, Net::X0, k)) || (r = O::forward(c, k)) || (r = O::download(c, Net::OUT, k, out + b0 * HH)))
+        return r;
+      return VW_OK;
+    });
+    if (rc) return rc;
+  }
+  return VW_OK;
+}
+
+/* vw_unet_predict without allocation: the FP32 staging of the images in and
+ * out goes through this device's scratch array (the kernels and the forward
+ * pass are those of vw_unet_predict).  Batches of batch_max images; the
+ * largest, min(batch_max, n) x img_n^2 values, must fit the array.  Calls on
+ * one device share the array: calls that overlap there race. */
+VW_API int vw_unet_predict_noalloc(void* p, const float* X, int32_t n, float* out) {
+  Net* c = as_net(p);
+  if (!c || !X || !out || n < 0) return VW_ERR_ARG;
+  if (!c->scratch_dev) return c->fail(VW_ERR_STATE, "scratch address not resolved");
+  const int64_t HH = (int64_t)c->N * c->N;
+  const int64_t kmax = std::min<int64_t>(c->BM, n);
+  if (kmax * HH > VW_UNET_SCRATCH_FLOATS)
+    return c->fail(VW_ERR_RANGE, "vw_unet_predict_noalloc: " + std::to_string(kmax) + " images of " +
+                                     std::to_string(c->N) + "^2 exceed the staging array of " +
+                                     std::to_string(VW_UNET_SCRATCH_FLOATS) + " values");
+  for (int32_t b0 = 0; b0 < n; b0 += c->BM) {
+    const int k = std::min(c->BM, n - b0);
+    int rc = dispatch(c, [&](auto ops) {
+      using O = decltype(ops);
+      int r;
+      if ((r = O::upload(c, X + b0 * HH, Net::X0, k, false, c->scratch_dev)) || (r = O::forward(c, k)) ||
+          (r = O::download(c, Net::OUT, k, out + b0 * HH, c->scratch_dev)))
         return r;
       return VW_OK;
     });

@@ -109,10 +109,13 @@ class UNet:
         self._check(self._lib.vw_unet_train_ms(self._ctx, C.byref(ms)))
         return ms.value
 
-    def predict(self, X):
+    def predict(self, X, noalloc=False):
+        """network output for X [n][img_n][img_n]; noalloc=True stages the
+        images through the library's static device array (no allocation)"""
         X = np.ascontiguousarray(X, np.float32)
         out = np.zeros_like(X)
-        self._check(self._lib.vw_unet_predict(self._ctx, _fp(X), X.shape[0], _fp(out)))
+        f = self._lib.vw_unet_predict_noalloc if noalloc else self._lib.vw_unet_predict
+        self._check(f(self._ctx, _fp(X), X.shape[0], _fp(out)))
         return out
 
 
