@@ -57,7 +57,7 @@ WORKLOADS = {
                  "circular orbits r in [6900,7100] km, 1,024 epochs x 10 s, 10 MHz, 16 segments, "
                  "blockage=1 occ_mode=2 uv-mirror recon, HEALPix map + 1024x1024 lm image"),
 }
-LEGS = ("nside128", "nside256", "batched128", "batched256", "cpu")
+LEGS = ("nside128", "nside256", "batched128", "batched256", "skysched128", "cpu")
 SKY_SEED = 20240
 BATCH_NSKY = 64
 
@@ -407,6 +407,78 @@ def batched_leg(workload, steps, warmup, serial, targets=None):
     }
 
 
+def sky_schedule_leg(workload, steps, warmup, serial):
+    """64 skies, each on its own uneven schedule (segment lengths uniform in
+    [16, 256] epochs, one seeded draw per sky): the batched Viss over the
+    context's 256-epoch segments shared by every sky, then per sky its own
+    groups, one-sky mixed recon and weights; maps of four skies against the
+    one-sky plain FP64 path on their schedules"""
+    from viss_optimize_ws_amd import Pipeline, make_sky
+
+    W, name = WORKLOADS[workload]
+    T = W["n_epochs"]
+    scheds = [_uneven_schedule(T, seed=4242 + s) for s in range(BATCH_NSKY)]
+    base = [256] * (T // 256) + ([T % 256] if T % 256 else [])
+    npix = 12 * W["nside"] ** 2
+    P = W["n_ant"] * (W["n_ant"] - 1) // 2
+    skies = np.stack([make_sky(7000 + s, npix, 1) for s in range(BATCH_NSKY)])
+    # keys per distinct segment (they depend on the geometry only)
+    segs = sorted({(t0, tl) for sc in scheds for t0, tl in zip(np.cumsum([0] + sc[:-1]).tolist(), sc)})
+    nkeys = {}
+    with Pipeline(_cfg(workload, True, seg_tlen=[256] * (T // 256) + ([T % 256] if T % 256 else []))) as p:
+        nimg = p.plan.n_img_pts
+        p.set_sky(skies[0])
+        for t0, tl in segs:
+            p.gen_segment(t0, tl)
+            p.viss_half()
+            p.conj_fill()
+            nkeys[(t0, tl)] = p.recon_segment()["nkeys"]
+        p.reset_accumulators()
+    terms = 0
+    for sc in scheds:
+        t0s = np.cumsum([0] + sc[:-1]).tolist()
+        terms += sum(npix * tl * P + (npix + nimg) * nkeys[(t0, tl)] for t0, tl in zip(t0s, sc))
+    ref = []
+    for s in range(4):
+        with Pipeline(_cfg(workload, serial, seg_tlen=scheds[s])) as p:
+            p.set_sky(skies[s])
+            p.run_all()
+            ref.append(p.finalize())
+    with Pipeline(_cfg(workload, serial, precision=1, nsky=BATCH_NSKY, seg_tlen=base)) as b:
+        b.set_sky(skies)
+        b.set_sky_schedules(scheds)
+        b.run_all()
+        Cb, Wb, Ib, WIb = b.finalize()
+        sec_eager, _ = _time_steps(b, steps, warmup)
+        sec_graph, _ = _time_steps(b, steps, max(warmup, 1), graph=True)
+        Cg, Wg, Ig, _ = b.finalize()
+        graph_equal = bool(np.array_equal(Cg, Cb) and np.array_equal(Ig, Ib) and np.array_equal(Wg, Wb))
+    sec = min(sec_eager, sec_graph)
+    with Pipeline(_cfg(workload, True, precision=1, nsky=BATCH_NSKY, seg_tlen=base)) as b2:
+        b2.set_sky(skies)
+        b2.set_sky_schedules(scheds)
+        _, tim = _time_steps(b2, steps, warmup)
+    err = {}
+    for key, k, got in (("C_healpix", 0, Cb), ("lm_image", 2, Ib)):
+        err[key] = {"max_rel": max(float(np.max(np.abs(got[s] - ref[s][k])) / np.max(np.abs(ref[s][k])))
+                                   for s in range(4)),
+                    "rms_rel": max(float(np.sqrt(np.mean((got[s] - ref[s][k]) ** 2)) / np.sqrt(np.mean(ref[s][k] ** 2)))
+                                   for s in range(4))}
+    weights_equal = bool(all(np.array_equal(Wb[s], ref[s][1]) and np.array_equal(WIb[s], ref[s][3]) for s in range(4)))
+    return {
+        "workload": name + f"; {BATCH_NSKY} skies, each on its own schedule of segments uniform in [16, 256] epochs",
+        "value": round(terms / sec / 1e9, 2), "unit": "Gterms/s", "ms_per_step": round(sec * 1e3, 4),
+        "ms_per_step_eager": round(sec_eager * 1e3, 4), "ms_per_step_graph": round(sec_graph * 1e3, 4),
+        "graph_equals_eager": graph_equal, "nsky": BATCH_NSKY, "terms_per_step": terms,
+        "segments": sum(len(x) for x in scheds), "distinct_segments": len(segs), "viss_schedule_epochs": base,
+        "schedules_first_4_skies": scheds[:4],
+        "dtype": "Viss: f64 phase, f32 hardware sincos, bf16 hi/lo matrix-core products; recon per sky: f64 phase, "
+                 "f32 hardware sincos, f32 block sums folded in f64",
+        "err_vs_fp64_4_skies": err, "weights_bitexact_4_skies": weights_equal,
+        "kernel_ms_per_step_serial": _kernel_ms(tim, steps),
+    }
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -592,6 +664,12 @@ def main():
         _log("leg batched128 done")
         if "nside128" in legs:
             out["batched"]["speedup_vs_plain"] = round(out["batched"]["value"] / out["value"], 3)
+    if "skysched128" in legs:
+        out["batched_sky_schedules"] = sky_schedule_leg("nside128", args.steps256, args.warmup256, args.serial)
+        _log("leg skysched128 done")
+        if out.get("batched"):
+            out["batched_sky_schedules"]["vs_shared_schedule"] = round(
+                out["batched_sky_schedules"]["value"] / out["batched"]["value"], 3)
     if "batched256" in legs:
         b = batched_leg("nside256", args.steps256, args.warmup256, args.serial, targets)
         _log("leg batched256 done")

@@ -249,6 +249,22 @@ int vw_get_timings(void* ctx, vw_timings* out);
  * and 16384).  Batched contexts only; no counterpart in the reference. */
 int vw_set_batch_targets(void* ctx, int64_t viss_waves, int64_t recon_waves);
 int vw_get_batch_targets(void* ctx, int64_t* viss_waves, int64_t* recon_waves);
+/* Per-sky segment schedules for a batched context (blockage = 1): sky s
+ * is segmented into nseg[s] consecutive segments whose lengths (epochs) are
+ * the next nseg[s] entries of seg_tlen, covering the run's epochs; every
+ * length must lie in [1, the longest segment of the context's own
+ * schedule].  vw_run_all and vw_run_all_graph then sum the visibilities of
+ * the context's schedule once for all skies (the half visibilities of an
+ * epoch do not depend on the segmentation), keep them for the whole day, and
+ * group and reconstruct each sky over its own segments (skies that share a
+ * segment share its groups), with the one-sky mixed recon kernel and one
+ * weight map per sky: vw_finalize then writes W_healpix [nsky][npix] and
+ * W_img [nsky][img_n^2].  vw_recon_segment and the accumulator get/set calls
+ * are refused in this mode.  nseg = NULL returns to the shared schedule.
+ * Allocates the day buffer (nsky x epochs x pairs x 16 bytes) on first use.
+ * No counterpart in the reference, which runs one sky. */
+int vw_set_sky_schedules(void* ctx, const int32_t* nseg, const int32_t* seg_tlen);
+
 /* Autotuner: times every target on the schedule's longest segment (reps
  * timed runs each, HIP events) and keeps the fastest for each kernel; the
  * accumulators are cleared afterwards.  Needs vw_set_sky. */

@@ -223,3 +223,76 @@ def test_batch_graph_replays():
         one.set_sky(np.ones(12 * 64))
         with pytest.raises(VissError):
             one.run_all_graph()
+
+
+# per-sky schedules: (context schedule, one schedule per sky); two skies
+# share their first segment, two share every segment
+SKY_SCHED_CASES = {
+    "small": (dict(nside=16, n_ant=8, n_epochs=40, img_n=32), [20, 20],
+              [[20, 20], [7, 13, 20], [20, 20], [1, 19, 20], [13, 7, 12, 8], [9, 9, 9, 9, 4], [7, 20, 13]]),
+    # segments above 8,192 rows (16 antennas x 70 epochs): the own radix sort
+    "long_rows": (dict(nside=8, n_ant=16, n_epochs=72, img_n=16), [70, 2], [[70, 2], [2, 70], [36, 36], [70, 2]]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(SKY_SCHED_CASES))
+def test_batch_per_sky_schedules(case):
+    """every sky on its own segment schedule: the maps and the per-sky weights
+    of the batched context equal the one-sky plain FP64 path run on that
+    sky's schedule (weights bit-exact, maps within 1e-4), from eager and
+    graph steps"""
+    kw, base, scheds = SKY_SCHED_CASES[case]
+    nsky = len(scheds)
+    npix = 12 * kw["nside"] ** 2
+    skies = _skies(nsky, npix, 71)
+    with Pipeline(config(precision=1, nsky=nsky, seg_tlen=base, **kw)) as g:
+        g.set_sky(skies)
+        g.set_sky_schedules(scheds)
+        g.run_all()
+        Cg, Wg, Ig, WIg = g.finalize()
+        assert Wg.shape == (nsky, npix) and WIg.shape == (nsky, kw["img_n"], kw["img_n"])
+        g.run_all_graph()
+        for a, b in zip(g.finalize(), (Cg, Wg, Ig, WIg)):
+            assert np.array_equal(a, b)
+        g.set_sky_schedules(None)  # back to the shared schedule
+        g.reset_accumulators()
+        g.run_all()
+        _, Wshared, _, _ = g.finalize()
+    for s in range(nsky):
+        with Pipeline(config(seg_tlen=scheds[s], **kw)) as p:
+            p.set_sky(skies[s])
+            p.run_all()
+            Cp, Wp, Ip, WIp = p.finalize()
+        assert np.array_equal(Wg[s], Wp) and np.array_equal(WIg[s], WIp), (case, s)
+        assert rel_err(Cg[s], Cp) <= MIXED_TOL, (case, s, rel_err(Cg[s], Cp))
+        assert rel_err(Ig[s], Ip) <= MIXED_TOL, (case, s, rel_err(Ig[s], Ip))
+        if scheds[s] == base:
+            assert np.array_equal(Wshared, Wp)
+
+
+def test_batch_per_sky_schedule_errors():
+    kw = dict(nside=8, n_ant=4, n_epochs=12, seg_tlen=[6, 6], img_n=8, precision=1, nsky=2)
+    with Pipeline(config(**kw)) as g:
+        g.set_sky(np.ones((2, 12 * 64)))
+        for bad, code in (([[6, 6], [12]], 4),       # a segment longer than the context's longest
+                          ([[6, 6], [6, 5]], 1),       # does not cover the run
+                          ([[6, 6], [6, 6, 1]], 1),    # beyond the run
+                          ([[6, 6], [0, 6, 6]], 4)):   # empty segment
+            with pytest.raises(VissError) as e:
+                g.set_sky_schedules(bad)
+            assert e.value.code == code, bad
+        g.set_sky_schedules([[6, 6], [3, 3, 6]])
+        g.gen_segment(0, 6)
+        g.viss_half()
+        g.conj_fill()
+        with pytest.raises(VissError):
+            g.recon_segment()  # per-sky recon runs in vw_run_all
+        with pytest.raises(VissError):
+            g.accumulators()
+    with Pipeline(config(nside=8, n_ant=4, n_epochs=4, segs=1, img_n=0)) as one:
+        with pytest.raises(VissError):
+            one.set_sky_schedules([[4]])
+    with Pipeline(config(nside=8, n_ant=4, n_epochs=12, seg_tlen=[6, 6], img_n=8, precision=1, nsky=2,
+                         blockage=0)) as b0:
+        with pytest.raises(VissError):
+            b0.set_sky_schedules([[6, 6], [6, 6]])

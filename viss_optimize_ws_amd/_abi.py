@@ -159,6 +159,7 @@ DEVICE_ONLY = {
     "get_batch_targets": (C.c_int, [P, PI64, PI64]),
     "autotune": (C.c_int, [P, i32, PI64, PI64]),
     "run_all_graph": (C.c_int, [P]),
+    "set_sky_schedules": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
     "make_sky": (C.c_int, [C.c_uint64, i64, i32, PD]),
     "io_load_sky": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, i64, PD, C.c_char_p, i32]),
     "io_write_map": (C.c_int, [C.c_char_p, PD, i64, i32]),

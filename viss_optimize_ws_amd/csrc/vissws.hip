@@ -137,6 +137,19 @@ struct Ctx {
   double* d_oant = nullptr;  /* its representative's antenna frames [position][10]    */
   int32_t* d_rshist[2] = {};  /* radix-sort digit tables per stream (Viss, grouping)   */
   int64_t rshist_cap = 0;
+  /* per-sky segment schedules (vw_set_sky_schedules): the batched Viss of
+   * the base schedule's segments is shared by every sky and kept for the
+   * whole day; each sky is then grouped and reconstructed over its own
+   * segments, with its own weights */
+  bool sky_sched = false;
+  struct SkySeg {
+    int t0, tlen;
+    std::vector<int32_t> skies; /* skies whose schedule holds this segment */
+  };
+  std::vector<SkySeg> sky_segs;  /* distinct (t0, tlen) of all skies, by t0 then tlen */
+  double2* d_vday = nullptr;      /* [nsky][T P] phase-corrected half visibilities of the day */
+  uint32_t *d_Wsky = nullptr, *d_GWsky = nullptr; /* per-sky weights [nsky][npix], [nsky][ngp] */
+  uint32_t* d_iwout_sky = nullptr; /* per-sky image weights [nsky][img_n^2] (finalize) */
   /* graph of a whole batched step (reset + every segment), captured on first use */
   hipGraphExec_t graph_exec = nullptr;
   bool capturing = false;
@@ -790,14 +803,14 @@ int enq_gen(Ctx* c, int t0, int tlen) {
 /* class table of a tile pass into the class scratch: cone bits per
  * (tile, antenna frame), then per (tile, half row) or (tile, group) */
 int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups,
-                uint8_t* bits, uint8_t* cls, hipStream_t st) {
+                uint8_t* bits, uint8_t* cls, hipStream_t st, bool work_order = false) {
   const int64_t nfr = (int64_t)c->tlen * c->A;
   if (nt * c->nfr_pad > c->bits_cap) return c->fail(VW_ERR_RANGE, "internal: cone bit buffer too small");
   k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad, bits);
   KC();
   if (groups)
     k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib, c->d_info,
-                                                            c->nq_pad, c->batch ? c->d_order[1] : nullptr, cls);
+                                                            c->nq_pad, work_order ? c->d_order[1] : nullptr, cls);
   else
     k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, cls);
   KC();
@@ -1001,12 +1014,11 @@ int enq_conj(Ctx* c) {
   return VW_OK;
 }
 
-int enq_group(Ctx* c) {
-  if (!c->have_conj) return c->fail(VW_ERR_STATE, "vw_recon_segment needs vw_conj_fill");
+/* w-plane fit, keys, stable order and runs of the current segment's rows
+ * (grouping stream); the groups depend on the geometry only */
+int enq_group_keys(Ctx* c) {
   bool full = c->cfg.recon_mode == VW_RECON_FULL;
   int64_t nr = full ? 2 * c->nh : c->nh;
-  HC(hipStreamWaitEvent(c->stB, c->ev_conj, 0));
-  ev_begin(c, ST_GROUP, c->stB);
   k_segparams<<<1, 256, 0, c->stB>>>(c->d_rows, c->nh, c->P, c->bl_max, c->lambda, c->d_info);
   KC();
   k_keys<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
@@ -1041,11 +1053,29 @@ int enq_group(Ctx* c) {
   }
   k_groups<<<1, 1024, 0, c->stB>>>(c->d_keys_sorted, nr, c->cfg.blockage, c->d_gkeys, c->d_goff, c->d_info);
   KC();
-  k_group_data<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, c->d_vh, c->d_rows,
+  return VW_OK;
+}
+
+/* group records (cell centre, representative or mean visibility from vh,
+ * antenna frames) of the current groups */
+int enq_group_data(Ctx* c, const double2* vh) {
+  bool full = c->cfg.recon_mode == VW_RECON_FULL;
+  int64_t nr = full ? 2 * c->nh : c->nh;
+  k_group_data<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_gkeys, c->d_goff, c->d_idx_sorted, vh, c->d_rows,
                                                   c->d_ant8, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode,
                                                   c->cfg.precision == VW_PREC_MIXED ? 1.0 : VW_TAB_SCALE, c->d_info,
                                                   c->gs);
   KC();
+  return VW_OK;
+}
+
+int enq_group(Ctx* c) {
+  if (!c->have_conj) return c->fail(VW_ERR_STATE, "vw_recon_segment needs vw_conj_fill");
+  bool full = c->cfg.recon_mode == VW_RECON_FULL;
+  HC(hipStreamWaitEvent(c->stB, c->ev_conj, 0));
+  ev_begin(c, ST_GROUP, c->stB);
+  int rcg;
+  if ((rcg = enq_group_keys(c)) || (rcg = enq_group_data(c, c->d_vh))) return rcg;
   if (c->batch) {
     int rc;
     if ((rc = enq_order(c, 1))) return rc;
@@ -1084,7 +1114,7 @@ int enq_recon_set_batch(Ctx* c, bool img) {
     const int64_t pt0 = tile0 * c->TP;
     const int64_t np = std::min(nt * c->TP, npts_all - pt0);
     if (rep) {
-      int rc = enq_classes(c, cone, ntile_all, tile0, nt, true, bits, cls, st);
+      int rc = enq_classes(c, cone, ntile_all, tile0, nt, true, bits, cls, st, true);
       if (rc) return rc;
     }
     const ReconBShape s = reconb_shape(c, np, c->rb_target);
@@ -1123,8 +1153,9 @@ int enq_recon_set_batch(Ctx* c, bool img) {
   return VW_OK;
 }
 
-int enq_recon_set(Ctx* c, bool img) {
-  if (c->batch) return enq_recon_set_batch(c, img);
+/* one-sky recon of one output set into the accumulators acc / wacc (per
+ * tile pass: classes, the recon kernel, the ordered fold of its chunks) */
+int enq_recon_set_one(Ctx* c, bool img, double* acc, uint32_t* wacc) {
   /* each output set has its own stream and scratch */
   hipStream_t st = img ? c->st2 : c->stB;
   uint8_t* cls = img ? c->d_cls2 : c->d_cls;
@@ -1189,10 +1220,15 @@ int enq_recon_set(Ctx* c, bool img) {
     KC();
     ev_end(c, ST_RECONK, st);
     k_recon_reduce<<<blocks_for(np), 256, 0, st>>>(a.part, a.wpart, a.nchunk, a.pstride, pt0, np, c->cfg.blockage,
-                                                      img ? c->d_Gacc : c->d_Cacc, img ? c->d_GWacc : c->d_Wacc);
+                                                      acc, wacc);
     KC();
   }
   return VW_OK;
+}
+
+int enq_recon_set(Ctx* c, bool img) {
+  if (c->batch) return enq_recon_set_batch(c, img);
+  return enq_recon_set_one(c, img, img ? c->d_Gacc : c->d_Cacc, img ? c->d_GWacc : c->d_Wacc);
 }
 
 int enq_recon(Ctx* c) {
@@ -1212,6 +1248,58 @@ int enq_recon(Ctx* c) {
   ev_end(c, ST_RECON, c->stB);
   /* the segment's buffer set is free for the segment after next */
   HC(hipEventRecord(c->ev_recon[c->cur], c->stB));
+  return VW_OK;
+}
+
+/* one step over the segment schedule (enqueue only): every segment's
+ * geometry, Viss, conjugate fill and recon; with per-sky schedules the
+ * Viss of the base segments for every sky into the day buffer, then per
+ * distinct (t0, tlen) of the skies' schedules the groups once and, for each
+ * sky holding that segment, its group records and recon into its own
+ * accumulators and weights */
+int enq_step(Ctx* c) {
+  int rc;
+  if (!c->sky_sched) {
+    for (int s = 0; s < c->segs; ++s)
+      if ((rc = enq_gen(c, c->seg_t0[s], c->seg_tlen[s])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c)) ||
+          (rc = enq_recon(c)))
+        return rc;
+    return VW_OK;
+  }
+  const int64_t day_rows = (int64_t)c->T * c->P;
+  for (int s = 0; s < c->segs; ++s) {
+    if ((rc = enq_gen(c, c->seg_t0[s], c->seg_tlen[s])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c))) return rc;
+    k_vday_store<<<blocks_for(c->nsky * c->nh), 256, 0, c->st>>>(c->d_vh, c->nsky, c->nh,
+                                                                  (int64_t)c->seg_t0[s] * c->P, day_rows, c->d_vday);
+    KC();
+  }
+  for (const Ctx::SkySeg& u : c->sky_segs) {
+    if ((rc = enq_gen(c, u.t0, u.tlen))) return rc;
+    /* the segment's rows and the whole day buffer are written on st */
+    HC(hipEventRecord(c->ev_conj, c->st));
+    HC(hipStreamWaitEvent(c->stB, c->ev_conj, 0));
+    ev_begin(c, ST_GROUP, c->stB);
+    if ((rc = enq_group_keys(c))) return rc;
+    ev_end(c, ST_GROUP, c->stB);
+    for (const int32_t sk : u.skies) {
+      if ((rc = enq_group_data(c, c->d_vday + sk * day_rows + (int64_t)u.t0 * c->P))) return rc;
+      ev_begin(c, ST_RECON, c->stB);
+      if (c->ngp) {
+        HC(hipEventRecord(c->ev_fork, c->stB));
+        HC(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+        if ((rc = enq_recon_set_one(c, true, c->d_Gacc + sk * c->ngp, c->d_GWsky + sk * c->ngp))) return rc;
+        HC(hipEventRecord(c->ev_join, c->st2));
+      }
+      if (c->cfg.healpix_out &&
+          (rc = enq_recon_set_one(c, false, c->d_Cacc + sk * c->npix, c->d_Wsky + sk * c->npix)))
+        return rc;
+      if (c->ngp) HC(hipStreamWaitEvent(c->stB, c->ev_join, 0));
+      ev_end(c, ST_RECON, c->stB);
+    }
+    /* this buffer set is free once the segment's last recon is done */
+    HC(hipEventRecord(c->ev_recon[c->cur], c->stB));
+  }
+  c->have_seg = c->have_viss = c->have_conj = c->have_groups = false;
   return VW_OK;
 }
 
@@ -1456,6 +1544,7 @@ VW_API int vw_conj_fill(void* p, vw_c128* oh, vw_c128* of) {
 VW_API int vw_recon_segment(void* p, vw_seg_info* info) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (c->sky_sched) return c->fail(VW_ERR_STATE, "with per-sky schedules the recon runs in vw_run_all");
   int rc = enq_recon(c);
   if (rc) return rc;
   if ((rc = sync(c))) return rc;
@@ -1465,13 +1554,9 @@ VW_API int vw_recon_segment(void* p, vw_seg_info* info) {
 VW_API int vw_run_all(void* p) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (c->sky_sched && !c->have_sky) return c->fail(VW_ERR_STATE, "vw_run_all needs vw_set_sky");
   int rc;
-  for (int s = 0; s < c->segs; ++s) {
-    if ((rc = enq_gen(c, c->seg_t0[s], c->seg_tlen[s])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c)) ||
-        (rc = enq_recon(c)))
-      return rc;
-  }
-  if ((rc = join_recon(c))) return rc;
+  if ((rc = enq_step(c)) || (rc = join_recon(c))) return rc;
   return sync(c);
 }
 
@@ -1492,6 +1577,14 @@ int enq_reset_fill(Ctx* c) {
   }
   k_fill0<<<1, 256, 0, c->st>>>((uint32_t*)c->d_info, (int64_t)(sizeof(DevInfo) / 4));
   KC();
+  if (c->sky_sched) {
+    k_fill0<<<blocks_for(c->npix * c->nsky), 256, 0, c->st>>>(c->d_Wsky, c->npix * c->nsky);
+    KC();
+    if (c->ngp) {
+      k_fill0<<<blocks_for(c->ngp * c->nsky), 256, 0, c->st>>>(c->d_GWsky, c->ngp * c->nsky);
+      KC();
+    }
+  }
   return VW_OK;
 }
 }  // namespace
@@ -1514,11 +1607,7 @@ VW_API int vw_run_all_graph(void* p) {
      * inside the capture */
     for (int b = 0; b < 2; ++b) (void)hipEventRecord(c->ev_recon[b], c->st);
     rc = enq_reset_fill(c);
-    for (int sg = 0; sg < c->segs && !rc; ++sg) {
-      if ((rc = enq_gen(c, c->seg_t0[sg], c->seg_tlen[sg])) || (rc = enq_viss(c, false)) || (rc = enq_conj(c)) ||
-          (rc = enq_recon(c)))
-        break;
-    }
+    if (!rc) rc = enq_step(c);
     if (!rc) rc = join_recon(c);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->st, &g);
@@ -1549,9 +1638,9 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
                                                         nullptr, c->d_out, c->d_wout);
     KC();
     if (c->batch) {
-      k_normalize_b<<<blocks_for(c->npix * c->nsky), 256, 0, c->st>>>(c->d_Cacc, c->d_Wacc, c->npix, c->nsky,
-                                                                        c->cfg.blockage, c->d_info, nullptr, c->npix,
-                                                                        c->d_out);
+      k_normalize_b<<<blocks_for(c->npix * c->nsky), 256, 0, c->st>>>(
+          c->d_Cacc, c->sky_sched ? c->d_Wsky : c->d_Wacc, c->npix, c->nsky, c->cfg.blockage, c->d_info, nullptr,
+          c->npix, c->d_out, c->sky_sched ? c->npix : 0);
       KC();
     }
   }
@@ -1563,9 +1652,9 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
                                                        c->d_gidx, c->d_iout, c->d_iwout);
     KC();
     if (c->batch) {
-      k_normalize_b<<<blocks_for(c->ngp * c->nsky), 256, 0, c->st>>>(c->d_Gacc, c->d_GWacc, c->ngp, c->nsky,
-                                                                       c->cfg.blockage, c->d_info, c->d_gidx, NN,
-                                                                       c->d_iout);
+      k_normalize_b<<<blocks_for(c->ngp * c->nsky), 256, 0, c->st>>>(
+          c->d_Gacc, c->sky_sched ? c->d_GWsky : c->d_GWacc, c->ngp, c->nsky, c->cfg.blockage, c->d_info, c->d_gidx,
+          NN, c->d_iout, c->sky_sched ? c->ngp : 0);
       KC();
     }
   }
@@ -1575,14 +1664,26 @@ VW_API int vw_finalize(void* p, double* C, uint32_t* W, double* img, uint32_t* W
       HC(hipMemcpyAsync(C, c->d_out, sizeof(double) * c->npix * c->nsky, hipMemcpyDeviceToHost, c->st));
     else std::memset(C, 0, sizeof(double) * c->npix * c->nsky);
   }
+  const int64_t nw = c->sky_sched ? c->nsky : 1; /* weight maps */
   if (W) {
-    if (c->cfg.healpix_out) HC(hipMemcpyAsync(W, c->d_wout, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
-    else std::memset(W, 0, sizeof(uint32_t) * c->npix);
+    if (c->cfg.healpix_out)
+      HC(hipMemcpyAsync(W, c->sky_sched ? c->d_Wsky : c->d_wout, sizeof(uint32_t) * c->npix * nw,
+                        hipMemcpyDeviceToHost, c->st));
+    else std::memset(W, 0, sizeof(uint32_t) * c->npix * nw);
   }
   if (c->ngp) {
     int64_t NN = (int64_t)c->cfg.img_n * c->cfg.img_n;
     if (img) HC(hipMemcpyAsync(img, c->d_iout, sizeof(double) * NN * c->nsky, hipMemcpyDeviceToHost, c->st));
-    if (Wimg) HC(hipMemcpyAsync(Wimg, c->d_iwout, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost, c->st));
+    if (Wimg && c->sky_sched) {
+      k_fill0<<<blocks_for(NN * c->nsky), 256, 0, c->st>>>(c->d_iwout_sky, NN * c->nsky);
+      KC();
+      k_scatter_w<<<blocks_for(c->ngp * c->nsky), 256, 0, c->st>>>(c->d_GWsky, c->ngp, c->nsky, c->d_gidx, NN,
+                                                                    c->d_iwout_sky);
+      KC();
+      HC(hipMemcpyAsync(Wimg, c->d_iwout_sky, sizeof(uint32_t) * NN * c->nsky, hipMemcpyDeviceToHost, c->st));
+    } else if (Wimg) {
+      HC(hipMemcpyAsync(Wimg, c->d_iwout, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost, c->st));
+    }
   }
   return sync(c);
 }
@@ -1598,12 +1699,17 @@ VW_API int vw_reset_accumulators(void* p) {
     HC(hipMemsetAsync(c->d_GWacc, 0, sizeof(uint32_t) * c->ngp, c->st));
   }
   HC(hipMemsetAsync(c->d_info, 0, sizeof(DevInfo), c->st));
+  if (c->sky_sched) {
+    HC(hipMemsetAsync(c->d_Wsky, 0, sizeof(uint32_t) * c->npix * c->nsky, c->st));
+    if (c->ngp) HC(hipMemsetAsync(c->d_GWsky, 0, sizeof(uint32_t) * c->ngp * c->nsky, c->st));
+  }
   return sync(c);
 }
 
 VW_API int vw_get_accumulators(void* p, double* C, uint32_t* W, double* G, uint32_t* GW, int64_t* wtotal) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;
+  if (c->sky_sched) return c->fail(VW_ERR_STATE, "accumulators of per-sky schedules are not exported");
   if (int rj = join_recon(c)) return rj;
   if (C) HC(hipMemcpyAsync(C, c->d_Cacc, sizeof(double) * c->npix * c->nsky, hipMemcpyDeviceToHost, c->st));
   if (W) HC(hipMemcpyAsync(W, c->d_Wacc, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost, c->st));
@@ -1619,6 +1725,7 @@ VW_API int vw_set_accumulators(void* p, const double* C, const uint32_t* W, cons
                                int64_t wtotal) {
   Ctx* c = as_ctx(p);
   if (!c || wtotal < 0) return VW_ERR_ARG;
+  if (c->sky_sched) return c->fail(VW_ERR_STATE, "accumulators of per-sky schedules are not imported");
   if (int rj = join_recon(c)) return rj;
   if (C) HC(hipMemcpyAsync(c->d_Cacc, C, sizeof(double) * c->npix * c->nsky, hipMemcpyHostToDevice, c->st));
   else HC(hipMemsetAsync(c->d_Cacc, 0, sizeof(double) * c->npix * c->nsky, c->st));
@@ -1665,6 +1772,64 @@ VW_API int vw_get_batch_targets(void* p, int64_t* viss_waves, int64_t* recon_wav
  * kernel and the recon stage for every launch target (reps runs each, HIP
  * events on the context's streams) and keeps the fastest.  The accumulators
  * are cleared afterwards. */
+VW_API int vw_set_sky_schedules(void* p, const int32_t* nseg, const int32_t* seg_tlen) {
+  Ctx* c = as_ctx(p);
+  if (!c) return VW_ERR_ARG;
+  if (!c->batch) return c->fail(VW_ERR_ARG, "per-sky schedules are for batched contexts (nsky > 1)");
+  int rc;
+  if ((rc = join_recon(c)) || (rc = sync(c))) return rc;
+  drop_graph(c);
+  if (!nseg) {
+    c->sky_sched = false;
+    c->sky_segs.clear();
+    return vw_reset_accumulators(c);
+  }
+  if (!seg_tlen) return VW_ERR_ARG;
+  if (!c->cfg.blockage) return c->fail(VW_ERR_ARG, "per-sky schedules need blockage = 1 (per-sky weights)");
+  /* validate every sky's schedule: lengths in [1, longest base segment],
+   * covering the run's epochs exactly */
+  std::vector<Ctx::SkySeg> segs;
+  int64_t off = 0;
+  for (int64_t sk = 0; sk < c->nsky; ++sk) {
+    if (nseg[sk] < 1 || nseg[sk] > c->T)
+      return c->fail(VW_ERR_ARG, "sky " + std::to_string(sk) + ": segment count out of range");
+    int t0 = 0;
+    for (int32_t k = 0; k < nseg[sk]; ++k) {
+      const int32_t tl = seg_tlen[off + k];
+      if (tl < 1 || tl > c->max_tlen)
+        return c->fail(VW_ERR_RANGE, "sky " + std::to_string(sk) + ": segment of " + std::to_string(tl) +
+                                         " epochs outside [1, " + std::to_string(c->max_tlen) +
+                                         "] (the longest segment of the context's schedule)");
+      if ((int64_t)t0 + tl > c->T) return c->fail(VW_ERR_ARG, "sky " + std::to_string(sk) + ": schedule beyond the run");
+      bool found = false;
+      for (Ctx::SkySeg& u : segs)
+        if (u.t0 == t0 && u.tlen == tl) {
+          u.skies.push_back((int32_t)sk);
+          found = true;
+          break;
+        }
+      if (!found) segs.push_back(Ctx::SkySeg{t0, tl, {(int32_t)sk}});
+      t0 += tl;
+    }
+    if (t0 != c->T) return c->fail(VW_ERR_ARG, "sky " + std::to_string(sk) + ": schedule does not cover the run");
+    off += nseg[sk];
+  }
+  std::sort(segs.begin(), segs.end(), [](const Ctx::SkySeg& a, const Ctx::SkySeg& b) {
+    return a.t0 != b.t0 ? a.t0 < b.t0 : a.tlen < b.tlen;
+  });
+  if (!c->d_vday) {
+    if ((rc = dalloc(c, &c->d_vday, (size_t)c->nsky * c->T * c->P)) ||
+        (rc = dalloc(c, &c->d_Wsky, (size_t)c->nsky * c->npix)))
+      return rc;
+    if (c->ngp && ((rc = dalloc(c, &c->d_GWsky, (size_t)c->nsky * c->ngp)) ||
+                   (rc = dalloc(c, &c->d_iwout_sky, (size_t)c->nsky * c->cfg.img_n * c->cfg.img_n))))
+      return rc;
+  }
+  c->sky_segs = std::move(segs);
+  c->sky_sched = true;
+  return vw_reset_accumulators(c);
+}
+
 VW_API int vw_autotune(void* p, int32_t reps, int64_t* viss_waves, int64_t* recon_waves) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;

@@ -700,20 +700,42 @@ __global__ void k_reconb_fold(const double* __restrict__ part, const uint32_t* _
   }
 }
 
-/* final maps of every sky: acc / W (blockage 1) or acc / W_total (blockage 0) */
+/* final maps of every sky: acc / W (blockage 1) or acc / W_total (blockage
+ * 0); the weights of sky s start at w + s * w_stride (0: one map for all) */
 __global__ void k_normalize_b(const double* __restrict__ acc, const uint32_t* __restrict__ w, int64_t n, int64_t nsky,
                               int blockage, const DevInfo* __restrict__ info, const int64_t* __restrict__ scatter,
-                              int64_t out_stride, double* __restrict__ out) {
+                              int64_t out_stride, double* __restrict__ out, int64_t w_stride) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n * nsky) return;
   const int64_t s = k / n, i = k - s * n;
   double v;
-  if (blockage) v = w[i] > 0 ? acc[k] / (double)w[i] : 0.0;
+  const uint32_t wi = w[s * w_stride + i];
+  if (blockage) v = wi > 0 ? acc[k] / (double)wi : 0.0;
   else {
     const int64_t wt = info->wtotal;
     v = acc[k] * (wt > 0 ? 1.0 / (double)wt : 0.0);
   }
   out[s * out_stride + (scatter ? scatter[i] : i)] = v;
+}
+
+/* per-sky weights of the lm-grid points [nsky][n] onto the image grid
+ * [nsky][out_stride] (points outside the disk stay as cleared) */
+__global__ void k_scatter_w(const uint32_t* __restrict__ w, int64_t n, int64_t nsky, const int64_t* __restrict__ scatter,
+                            int64_t out_stride, uint32_t* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n * nsky) return;
+  const int64_t s = k / n, i = k - s * n;
+  out[s * out_stride + scatter[i]] = w[k];
+}
+
+/* every sky's phase-corrected half visibilities of the current segment
+ * [nsky][nh] into the day buffer [nsky][day_rows] at row offset row0 */
+__global__ void k_vday_store(const double2* __restrict__ vh, int64_t nsky, int64_t nh, int64_t row0, int64_t day_rows,
+                             double2* __restrict__ vday) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsky * nh) return;
+  const int64_t s = k / nh, h = k - s * nh;
+  vday[s * day_rows + row0 + h] = vh[k];
 }
 
 /* zero fill of 32-bit words (kept out of hipMemsetAsync so that the batched

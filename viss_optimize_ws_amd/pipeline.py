@@ -169,12 +169,13 @@ class Stages:
     def finalize(self):
         npix = self.plan.npix
         Cm = self._stack(npix)
-        W = np.zeros(npix, np.uint32)
+        per_sky = getattr(self, "per_sky_weights", False)
+        W = np.zeros((self.nsky, npix) if per_sky else npix, np.uint32)
         nimg = self.cfg.img_n
         img = None
         if nimg:
             img = np.zeros((self.nsky, nimg, nimg) if self.nsky > 1 else (nimg, nimg))
-        Wi = np.zeros((nimg, nimg), np.uint32) if nimg else None
+        Wi = (np.zeros((self.nsky, nimg, nimg) if per_sky else (nimg, nimg), np.uint32)) if nimg else None
         self._check(self._f("finalize")(self._ctx, _ptr(Cm), _ptr(W, C.c_uint32), _ptr(img), _ptr(Wi, C.c_uint32)))
         return Cm, W, img, Wi
 
@@ -305,6 +306,19 @@ class Pipeline(Stages):
         v, r = C.c_int64(0), C.c_int64(0)
         self._check(self._lib.vw_autotune(self._ctx, int(reps), C.byref(v), C.byref(r)))
         return v.value, r.value
+
+    def set_sky_schedules(self, schedules):
+        """per-sky segment lengths (a list of epoch lists, one per sky) for a
+        batched context; None returns to the shared schedule.  vw_finalize
+        then returns one weight map per sky."""
+        if schedules is None:
+            self._check(self._lib.vw_set_sky_schedules(self._ctx, None, None))
+            self.per_sky_weights = False
+            return
+        nseg = np.array([len(x) for x in schedules], np.int32)
+        tl = np.array([t for x in schedules for t in x], np.int32)
+        self._check(self._lib.vw_set_sky_schedules(self._ctx, _ptr(nseg, C.c_int32), _ptr(tl, C.c_int32)))
+        self.per_sky_weights = True
 
     def timings(self):
         t = A.vw_timings()
