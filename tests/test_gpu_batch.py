@@ -73,7 +73,7 @@ CASES = {
     # a small class budget: several tile passes per stage, one block per wave
     "tile_passes": dict(nside=32, n_ant=8, n_epochs=16, segs=2, img_n=48, nsky=16, class_budget_bytes=64 * 1024),
     # a segment of 8,400 rows: grouping by the batched contexts' own radix sort
-    # (the one-sky path sorts with rocPRIM; equal weights mean equal groups)
+    # (equal weights mean equal groups)
     "long_segment": dict(nside=16, n_ant=16, n_epochs=72, img_n=32, nsky=20, seg_tlen=[70, 2]),
     # 70 skies: two sky groups of four 16-sky blocks (128 padded columns)
     "two_sky_groups": dict(nside=16, n_ant=8, n_epochs=16, segs=2, img_n=32, nsky=70),

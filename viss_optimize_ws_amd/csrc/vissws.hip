@@ -7,7 +7,6 @@
  * enqueue work; the public entry points synchronise before returning.
  */
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -90,8 +89,6 @@ struct Ctx {
   uint8_t* d_bits = nullptr; /* cone bits per (tile of a pass, antenna frame)            */
   int64_t nfr_pad = 0;       /* antenna frames of the longest segment, padded            */
   int64_t bits_cap = 0;      /* bytes of d_bits                                          */
-  void* d_sort_tmp = nullptr; /* radix sort scratch for segments above VW_RANK_MAX rows  */
-  size_t sort_tmp_bytes = 0;
   double2* d_part = nullptr;
   int64_t* d_cpart = nullptr;
   /* antenna-factorized Viss: per-frame phase coefficients and the partial
@@ -741,13 +738,11 @@ int create_device(Ctx* c) {
       if (nta > 0)
         for (int64_t w : {c->nh_pad, c->nq_pad}) maxnt = std::max(maxnt, recon_pass_tiles(c, nta, w));
     }
-    if (2 * c->max_nh > VW_RANK_MAX) {
-      HC(rocprim::radix_sort_pairs(nullptr, c->sort_tmp_bytes, (const int64_t*)nullptr, (int64_t*)nullptr,
-                                   (const int32_t*)nullptr, (int32_t*)nullptr, (unsigned)(2 * c->max_nh), 0u, 64u,
-                                   c->st));
-      uint8_t* tmp = nullptr;
-      if ((rc = dalloc(c, &tmp, c->sort_tmp_bytes))) return rc;
-      c->d_sort_tmp = tmp;
+    if (!c->batch && 2 * c->max_nh > VW_RANK_MAX) {
+      /* digit tables of the radix sort of long segments' (key, row) pairs
+       * (batched contexts size theirs for the work orders too) */
+      c->rshist_cap = 256 * ((2 * c->max_nh + VW_RS_BLOCK - 1) / VW_RS_BLOCK);
+      if ((rc = dalloc(c, &c->d_rshist[1], (size_t)c->rshist_cap + 256))) return rc; /* + digit totals */
     }
     c->nfr_pad = ((int64_t)c->max_tlen * c->A + 63) / 64 * 64;
     c->bits_cap = maxnt * c->nfr_pad;
@@ -1024,19 +1019,12 @@ int enq_group_keys(Ctx* c) {
   k_keys<<<blocks_for(nr), 256, 0, c->stB>>>(c->d_rows, nr, c->P, full ? 1 : 0, c->d_info, c->d_keys);
   KC();
   if (nr > VW_RANK_MAX) {
-    /* long segments: stable LSD radix sort of (key, row index) pairs; batched
-     * contexts use the memset-free sort so that their steps can be captured */
+    /* long segments: stable LSD radix sort of the (key, row index) pairs
+     * over the eight bytes of the non-negative int64 keys */
     k_iota<<<blocks_for(nr), 256, 0, c->stB>>>(nr, c->d_rank);
     KC();
-    if (c->batch) {
-      int rc = enq_rsort<int64_t, int32_t>(c, 1, c->stB, c->d_keys, c->d_rank, c->d_keys_sorted, c->d_idx_sorted, nr,
-                                           8);
-      if (rc) return rc;
-    } else {
-      size_t tb = c->sort_tmp_bytes;
-      HC(rocprim::radix_sort_pairs(c->d_sort_tmp, tb, (const int64_t*)c->d_keys, c->d_keys_sorted,
-                                   (const int32_t*)c->d_rank, c->d_idx_sorted, (unsigned)nr, 0u, 64u, c->stB));
-    }
+    int rc = enq_rsort<int64_t, int32_t>(c, 1, c->stB, c->d_keys, c->d_rank, c->d_keys_sorted, c->d_idx_sorted, nr, 8);
+    if (rc) return rc;
   } else {
     /* stable rank by (key, row): a grid of (256 rows) x (jblk rows) compare
      * tiles sized for about 2,048 workgroups, then a scatter */
