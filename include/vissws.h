@@ -200,6 +200,15 @@ int vw_gen_segment(void* ctx, int32_t t0, int32_t tlen, vw_geom_host* opt_out);
  * NULL) receives the raw (not phase-corrected) sums. */
 int vw_viss_half(void* ctx, vw_c128* opt_out);
 
+/* Adjoint of the half-visibility sum of the current segment (one-sky
+ * contexts): for residual rows r (raw convention of vw_viss_half, n_half
+ * entries) writes g[p] = pix_area * sum_h M_hp Re(r_h exp(+j 2 pi (u l + v m
+ * + w (n-1)))) for every pixel (npix entries), with the Viss's own mask M
+ * and FP64 phase and sincos: the gradient of 1/2 |V(B) - r_target|^2 with
+ * respect to the sky B is g(V(B) - r_target).  For data-consistency
+ * refinement of sky estimates; no counterpart in the reference. */
+int vw_viss_adjoint(void* ctx, const vw_c128* res, double* g);
+
 /* Phase correction Vh *= exp(-j 2 pi w) and conjugate completion
  * V[2P g + j] = Vh, V[2P g + P + j] = conj(Vh).  Replaces
  * phase_correct_viss_halfsym_host + expand_viss_halfsym_to_full

@@ -49,6 +49,12 @@ class Oracle(Stages):
                                            _ptr(cnt, C.c_int64)))
         return (out, cnt) if counts else out
 
+    def viss_adjoint(self, res):
+        res = np.ascontiguousarray(res, np.complex128)
+        g = np.zeros(self.plan.npix)
+        self._check(self._lib.vo_viss_adjoint(self._ctx, _ptr(res.view(np.float64)), _ptr(g)))
+        return g
+
     def set_viss_raw(self, raw):
         raw = np.ascontiguousarray(raw, np.complex128)
         self._check(self._lib.vo_set_viss_raw(self._ctx, _ptr(raw.view(np.float64))))

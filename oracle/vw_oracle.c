@@ -505,6 +505,35 @@ int vo_viss_half(void* ctx, int64_t row0, int64_t nrows, vw_c128* out, int64_t* 
   return VW_OK;
 }
 
+/* Adjoint of vo_viss_half over all rows of the current segment: per pixel,
+ * g_p = pix_area * sum_h M_hp Re(r_h exp(+j 2 pi (u l + v m + w (n-1))))
+ * with the same effective mask (tile class, then the predicate on partly
+ * visible tiles) and table sincos as the forward sum.  No reference
+ * counterpart (a data-consistency operator of this build's deconvolver);
+ * pinned by the adjoint identity <V(B), r> = <B, g(r)> against vo_viss_half. */
+int vo_viss_adjoint(void* ctx, const vw_c128* res, double* g) {
+  vo_ctx* c = (vo_ctx*)ctx;
+  if (!c->have_seg) return fail(c, VW_ERR_STATE, "need segment");
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t t = 0; t < c->ntile; ++t) {
+    int64_t p1 = (t + 1) * c->TP < c->npix ? (t + 1) * c->TP : c->npix;
+    for (int64_t p = t * c->TP; p < p1; ++p) g[p] = 0.0;
+    for (int64_t h = 0; h < c->nh; ++h) {
+      int cls = pair_class(c, h, c->cone, c->ntile, t);
+      if (cls == VW_CLS_HIDDEN) continue;
+      double u = c->ru[h], v = c->rv[h], w = c->rw[h];
+      for (int64_t p = t * c->TP; p < p1; ++p) {
+        if (cls == VW_CLS_MIXED && !predicate(c, h, c->l[p], c->m[p], c->n[p])) continue;
+        double s, co;
+        vw_sincos_tab(vw_viss_phase(u, v, w, c->l[p], c->m[p], c->nm1[p]) * VW_TAB_SCALE, g_tab, &s, &co);
+        g[p] = g[p] + (res[h].re * co - res[h].im * s);
+      }
+    }
+    for (int64_t p = t * c->TP; p < p1; ++p) g[p] = c->pix_area * g[p];
+  }
+  return VW_OK;
+}
+
 int vo_set_viss_raw(void* ctx, const vw_c128* raw) {
   vo_ctx* c = (vo_ctx*)ctx;
   if (!c->have_seg) return fail(c, VW_ERR_STATE, "need segment");

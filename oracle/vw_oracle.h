@@ -38,6 +38,8 @@ int vo_gen_segment(void* ctx, int32_t t0, int32_t tlen, vw_geom_host* opt_out);
 /* rows [row0, row0+nrows) of the current segment (nrows < 0: all).  counts
  * (may be NULL) receives the number of pixels summed per row. */
 int vo_viss_half(void* ctx, int64_t row0, int64_t nrows, vw_c128* out_raw, int64_t* counts);
+/* Adjoint of vo_viss_half over the current segment's rows: g [npix]. */
+int vo_viss_adjoint(void* ctx, const vw_c128* res, double* g);
 /* Replace the current segment's raw half sums (e.g. with device values). */
 int vo_set_viss_raw(void* ctx, const vw_c128* raw);
 int vo_conj_fill(void* ctx, vw_c128* opt_half_out, vw_c128* opt_full_out);

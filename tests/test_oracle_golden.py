@@ -254,3 +254,32 @@ def test_sincos_turn_accuracy(form):
     assert np.array_equal(sq, [0.0, 1.0, 0.0, -1.0, 0.0, -1.0, 0.0])
     assert np.array_equal(cq, [1.0, 0.0, -1.0, 0.0, 1.0, 0.0, -1.0])
     assert np.abs(s * s + c * c - 1).max() <= 6e-16
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(tile_pix=512), dict(geometry=0, nside=8, freq_hz=1e6, segs=20)])
+def test_oracle_viss_adjoint_identity(variant):
+    """the oracle's adjoint of the half-visibility sum satisfies
+    <V(B), r> = <B, g(r)> (real inner products) against its forward sum, on
+    random skies and residuals: a wrong sign, mask, phase or pixel-area factor
+    breaks it"""
+    kw = dict(nside=16, n_ant=8, n_epochs=8, segs=2, img_n=0)
+    kw.update(variant)
+    geom = kw.pop("geometry", 1)
+    if geom == 0:
+        kw.pop("n_ant"), kw.pop("n_epochs")
+    cfg = default_config(geom, **kw)
+    rng = np.random.default_rng(17)
+    with Oracle(cfg) as o:
+        B = rng.lognormal(0.0, 1.0, o.plan.npix)
+        o.set_sky(B)
+        t0, tl = o.schedule()[1]
+        o.gen_segment(t0, tl)
+        V = o.viss_half()
+        r = rng.normal(size=o.nh) + 1j * rng.normal(size=o.nh)
+        g = o.viss_adjoint(r)
+        lhs = float(np.sum(V.real * r.real + V.imag * r.imag))
+        rhs = float(np.dot(B, g))
+        assert abs(lhs - rhs) <= 1e-12 * (np.sum(np.abs(V) * np.abs(r)))
+        # masked pixels: hidden from every row of the segment, zero gradient
+        m = o.masks(0, o.nh).any(axis=0)
+        assert np.all(g[~m] == 0.0) and np.any(g[m] != 0.0)

@@ -149,6 +149,7 @@ STAGE_FUNCS = {
 DEVICE_ONLY = {
     "config_init": (C.c_int, [C.POINTER(vw_config), i32]),
     "viss_half": (C.c_int, [P, PD]),
+    "viss_adjoint": (C.c_int, [P, PD, PD]),
     "run_all": (C.c_int, [P]),
     "set_day": (C.c_int, [P, i32]),
     "get_timings": (C.c_int, [P, C.POINTER(vw_timings)]),
@@ -192,6 +193,7 @@ ORACLE_ONLY = {
     "group_segment": (C.c_int, [P, C.POINTER(vw_seg_info)]),
     "recon_points": (C.c_int, [P, i32, PI64, i64, PD, PU32]),
     "get_predicate": (C.c_int, [P, i64, i64, PU8]),
+    "viss_adjoint": (C.c_int, [P, PD, PD]),
 }
 
 

@@ -147,6 +147,10 @@ struct Ctx {
   double2* d_vday = nullptr;      /* [nsky][T P] phase-corrected half visibilities of the day */
   uint32_t *d_Wsky = nullptr, *d_GWsky = nullptr; /* per-sky weights [nsky][npix], [nsky][ngp] */
   uint32_t* d_iwout_sky = nullptr; /* per-sky image weights [nsky][img_n^2] (finalize) */
+  /* adjoint Viss (vw_viss_adjoint): residual rows, chunk partials, output */
+  double2* d_adj_res = nullptr;
+  double *d_adj_part = nullptr, *d_adj_out = nullptr;
+  int64_t adj_part_cap = 0;
   /* graph of a whole batched step (reset + every segment), captured on first use */
   hipGraphExec_t graph_exec = nullptr;
   bool capturing = false;
@@ -1760,6 +1764,51 @@ VW_API int vw_get_batch_targets(void* p, int64_t* viss_waves, int64_t* recon_wav
  * kernel and the recon stage for every launch target (reps runs each, HIP
  * events on the context's streams) and keeps the fastest.  The accumulators
  * are cleared afterwards. */
+/* waves per adjoint launch that the row chunking aims at */
+#define VW_ADJ_TARGET_WAVES 16384
+
+VW_API int vw_viss_adjoint(void* p, const vw_c128* res, double* g) {
+  Ctx* c = as_ctx(p);
+  if (!c || !res || !g) return VW_ERR_ARG;
+  if (c->batch) return c->fail(VW_ERR_ARG, "the adjoint runs on one-sky contexts");
+  if (!c->have_seg) return c->fail(VW_ERR_STATE, "vw_viss_adjoint needs vw_gen_segment");
+  int rc;
+  if ((rc = join_recon(c)) || (rc = sync(c))) return rc;
+  const int64_t nsub_max = c->vtpp * c->TP / 256;
+  if (!c->d_adj_res) {
+    c->adj_part_cap = (VW_ADJ_TARGET_WAVES + 2 * nsub_max) * 256;
+    if ((rc = dalloc(c, &c->d_adj_res, (size_t)c->max_nh)) || (rc = dalloc(c, &c->d_adj_out, (size_t)c->npix)) ||
+        (rc = dalloc(c, &c->d_adj_part, (size_t)c->adj_part_cap)))
+      return rc;
+  }
+  HC(hipMemcpyAsync(c->d_adj_res, res, sizeof(vw_c128) * c->nh, hipMemcpyHostToDevice, c->st));
+  for (int64_t tile0 = 0; tile0 < c->ntile; tile0 += c->vtpp) {
+    const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
+    if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
+    VissAdjArgs a{};
+    a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.npix = c->npix;
+    a.tile0 = tile0; a.ntile = nt;
+    a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls_v;
+    a.nh = c->nh; a.nh_pad = c->nh_pad; a.res = c->d_adj_res;
+    a.nsub = nt * c->TP / 256;
+    const int64_t want = std::max<int64_t>(1, VW_ADJ_TARGET_WAVES / a.nsub);
+    a.rpc = std::max<int64_t>(16, ((c->nh + want - 1) / want + 3) / 4 * 4);
+    a.nchunk = (c->nh + a.rpc - 1) / a.rpc;
+    if (a.nchunk * a.nsub * 256 > c->adj_part_cap) return c->fail(VW_ERR_RANGE, "internal: adjoint partials too small");
+    a.part = c->d_adj_part;
+    const unsigned nb = (unsigned)((a.nchunk * a.nsub + 3) / 4);
+    if (c->TP == 256) k_viss_adj<256><<<nb, 256, 0, c->st>>>(a);
+    else k_viss_adj<512><<<nb, 256, 0, c->st>>>(a);
+    KC();
+    const int64_t p0 = tile0 * c->TP, np = std::min(nt * c->TP, c->npix - p0);
+    k_viss_adj_fold<<<blocks_for(np), 256, 0, c->st>>>(c->d_adj_part, a.nchunk, a.nsub * 256, p0, np, c->pix_area,
+                                                       c->d_adj_out);
+    KC();
+  }
+  HC(hipMemcpyAsync(g, c->d_adj_out, sizeof(double) * c->npix, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
 VW_API int vw_set_sky_schedules(void* p, const int32_t* nseg, const int32_t* seg_tlen) {
   Ctx* c = as_ctx(p);
   if (!c) return VW_ERR_ARG;

@@ -507,6 +507,112 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
   }
 }
 
+/* adjoint of the plain Viss sum (a real-linear map from pixel weights to
+ * raw half visibilities): for one tile pass and one chunk of half rows,
+ * g_p = sum_h M_hp Re(r_h exp(+j 2 pi (u l + v m + w (n-1)))) per pixel,
+ * with the effective mask M of the Viss (tile class, then the per-pixel
+ * predicate on partly visible tiles) and its table sincos.  A wave owns one
+ * 256-pixel sub-tile (4 pixels per lane) and a chunk of rows; rows are
+ * wave-uniform, their class one byte of a uniform class word.  Partials
+ * [chunk][pixel of the pass]; the fold applies the pixel area. */
+struct VissAdjArgs {
+  const double *l, *m, *n, *nm1;
+  int64_t npix;
+  int64_t tile0, ntile; /* tiles of this pass: [tile0, tile0 + ntile) */
+  const RowRec* rows;
+  const double4* rowt; /* (u, v, w) * 1024 per half row */
+  const double* ant8;
+  const uint32_t* cls; /* [ntile][nh_pad] bytes (pass-relative) viewed as words */
+  int64_t nh, nh_pad;
+  const double2* res;  /* [nh] residual half visibilities (raw convention) */
+  int64_t rpc, nchunk; /* rows per chunk (a multiple of 4), chunks */
+  int64_t nsub;        /* 256-pixel sub-tiles of the pass */
+  double* part;        /* [nchunk][nsub * 256] */
+};
+
+template <int TP>
+__global__ __launch_bounds__(256) void k_viss_adj(VissAdjArgs a) {
+  constexpr int PPL = 4;
+  __shared__ double2 tab[VW_TAB_N];
+  vw_tab_fill(tab);
+  __syncthreads();
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int64_t wv = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t chunk = wv / a.nsub, sub = wv - chunk * a.nsub;
+  if (chunk >= a.nchunk) return;
+  const int64_t p0 = a.tile0 * TP + sub * 256; /* first pixel of the sub-tile */
+  const int64_t tile = p0 / TP;
+  const int64_t h0 = chunk * a.rpc;
+  const int64_t h1 = h0 + a.rpc < a.nh ? h0 + a.rpc : a.nh;
+  double pl[PPL], pm[PPL], pn[PPL], pz[PPL], acc[PPL];
+  bool ok[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    const int64_t p = p0 + lane + 64 * k;
+    ok[k] = p < a.npix;
+    const int64_t pp = ok[k] ? p : 0;
+    pl[k] = a.l[pp];
+    pm[k] = a.m[pp];
+    pn[k] = a.n[pp];
+    pz[k] = a.nm1[pp];
+    acc[k] = 0.0;
+  }
+  const uint32_t* cwp = a.cls + (((tile - a.tile0) * a.nh_pad) >> 2);
+  for (int64_t h4 = h0; h4 < h1; h4 += 4) {
+    const uint32_t cw = (uint32_t)vw_rfl((int)cwp[h4 >> 2]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t h = h4 + j;
+      const int cls = (int)((cw >> (8 * j)) & 0xffu);
+      if (h >= h1 || cls == VW_CLS_HIDDEN) continue;
+      const double2 r = a.res[h];
+      bool mk[PPL];
+      if (cls == VW_CLS_VISIBLE) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) mk[k] = ok[k];
+      } else {
+        const RowRec rec = a.rows[h];
+        const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+        const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+          mk[k] = vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], qa[0], qa[1], qa[2], qa[3], qa[7], qb[0], qb[1], qb[2],
+                                qb[3], qb[7]);
+      }
+      const double4 rt = a.rowt[h];
+      double f[PPL];
+      int ix[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) f[k] = vw_tab_split(vw_viss_phase(rt.x, rt.y, rt.z, pl[k], pm[k], pz[k]), &ix[k]);
+      double2 e[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        double sy, cy, sn, cs;
+        vw_tab_poly(f[k], &sy, &cy);
+        vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
+        /* Re(r exp(+j x)) = r.x cos x - r.y sin x */
+        const double t = fma(r.x, cs, fma(-r.y, sn, acc[k]));
+        acc[k] = mk[k] ? t : acc[k];
+      }
+    }
+  }
+  const int64_t stride = a.nsub * 256;
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) a.part[chunk * stride + sub * 256 + lane + 64 * k] = acc[k];
+}
+
+/* adjoint partials of a pass -> g[p] = area * sum over chunks (ascending) */
+__global__ void k_viss_adj_fold(const double* __restrict__ part, int64_t nchunk, int64_t stride, int64_t p0,
+                                int64_t np, double area, double* __restrict__ g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double s = 0.0;
+  for (int64_t k = 0; k < nchunk; ++k) s = s + part[k * stride + i];
+  g[p0 + i] = area * s;
+}
+
 /* partials -> raw half sums, chunks folded in ascending order; passes after
  * the first add to the running sums */
 __global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, int64_t nh, int64_t nh_pad,

@@ -14,11 +14,14 @@ The reference stops at the dirty map; this is the learned stage beside it:
 * ``normalise`` scales each pair by its dirty image's RMS inside the disk
   (and one global gain for the target), so the network sees O(1) values;
   ``denormalise`` undoes it.
-* ``resimulate_gate`` re-simulates the half visibilities of an estimate with
-  the plain Viss kernel (grid values put back on the HEALPix pixels of the
-  visible hemisphere) and compares them with the visibilities of the true
-  sky; an estimate whose relative residual exceeds the threshold (1e-2) is
-  replaced by the dirty image.
+* ``resimulate_gate`` puts an estimate back on the HEALPix pixels of the
+  visible hemisphere, optionally refines it towards the measured half
+  visibilities (``refine``: conjugate-gradient least squares with the plain
+  Viss kernel and its adjoint, ``vw_viss_adjoint``, restricted to the
+  visible hemisphere), re-simulates its half visibilities with the plain Viss
+  kernel and compares them with the measured ones; an estimate whose
+  relative residual exceeds the threshold (1e-2) is replaced by the dirty
+  image under the training set's gain (information available at inference).
 
 Command line (one MI355X): ``python -m viss_optimize_ws_amd.deconv --pairs
 20000 --heldout 512`` prints one JSON line with the training time, the
@@ -226,33 +229,100 @@ def grid_to_healpix(img, idx_pix, npix):
     return np.where(cnt > 0, out / np.maximum(cnt, 1.0), 0.0)
 
 
-def resimulate_gate(estimates, dirty, skies, cfg, threshold=GATE_THRESHOLD):
-    """visibility residual of each estimate (plain Viss of the estimate put on
-    the HEALPix grid against the plain Viss of the true sky, all segments):
-    returns (residuals, accepted mask, gated output = estimate or dirty)"""
+def healpix_to_grid(sky, idx_pix):
+    """HEALPix values sampled on the lm grid (the pixel of each grid
+    direction; 0 outside the unit disk), the targets' convention"""
+    return np.where(idx_pix >= 0, sky[np.maximum(idx_pix, 0)], 0.0)
+
+
+class Measurement:
+    """the plain FP64 half-visibility sums of the pairs' geometry as a linear
+    map of the sky, on one pipeline context: ``forward`` (all segments' raw
+    half visibilities) and ``adjoint`` (vw_viss_adjoint summed over the
+    segments)"""
+
+    def __init__(self, pipe):
+        self.p = pipe
+        self.sched = pipe.schedule()
+
+    def forward(self, x):
+        self.p.set_sky(x)
+        out = []
+        for t0, tl in self.sched:
+            self.p.gen_segment(t0, tl)
+            out.append(self.p.viss_half())
+        return out
+
+    def adjoint(self, rs):
+        g = np.zeros(self.p.plan.npix)
+        for (t0, tl), r in zip(self.sched, rs):
+            self.p.gen_segment(t0, tl)
+            g += self.p.viss_adjoint(r)
+        return g
+
+
+def _dot(a, b):
+    return float(sum(np.sum(x.real * y.real + x.imag * y.imag) for x, y in zip(a, b)))
+
+
+def refine(meas, x0, v, support, iters=50, tol=GATE_THRESHOLD * 0.5):
+    """data-consistency refinement: conjugate-gradient least squares (CGLS)
+    on min |A x - v|^2 over skies x supported on `support` (A = the plain
+    half-visibility sums of every segment), started from x0; stops once the
+    relative residual |A x - v| / |v| is at most tol.  Returns (x, relative
+    residual after each iteration)."""
+    x = np.where(support, x0, 0.0)
+    vn = np.sqrt(_dot(v, v))
+    r = [a - b for a, b in zip(v, meas.forward(x))]
+    hist = [np.sqrt(_dot(r, r)) / vn]
+    s = meas.adjoint(r) * support
+    d = s.copy()
+    gam = float(np.dot(s, s))
+    for _ in range(iters):
+        if hist[-1] <= tol or gam == 0.0:
+            break
+        q = meas.forward(d)
+        alpha = gam / _dot(q, q)
+        x = x + alpha * d
+        r = [a - alpha * b for a, b in zip(r, q)]
+        hist.append(np.sqrt(_dot(r, r)) / vn)
+        s = meas.adjoint(r) * support
+        g2 = float(np.dot(s, s))
+        d = s + (g2 / gam) * d
+        gam = g2
+    return x, hist
+
+
+def resimulate_gate(estimates, fallback, skies, cfg, threshold=GATE_THRESHOLD, refine_iters=0):
+    """re-simulation gate over grid estimates: each goes back onto the HEALPix
+    pixels of the visible hemisphere (grid_to_healpix), is refined towards
+    the measured visibilities when refine_iters > 0, and is re-simulated by
+    the plain Viss kernel; its relative residual against the visibilities of
+    the true sky decides.  Returns (residuals, accepted mask, gated grid
+    output = the (refined) estimate or the fallback image)."""
     idx = grid_pixel_index(cfg.nside, cfg.img_n)
     res = np.zeros(len(estimates))
+    out = np.array(estimates, dtype=np.float64)
     with Pipeline(cfg) as p:
         npix = p.plan.npix
+        support = p.lmn()[2] > 0.0
+        meas = Measurement(p)
         for k in range(len(estimates)):
-            est = grid_to_healpix(estimates[k], idx, npix)
-            num = den = 0.0
-            for t0, tl in p.schedule():
-                p.gen_segment(t0, tl)
-                p.set_sky(skies[k])
-                vt = p.viss_half()
-                p.set_sky(est)
-                ve = p.viss_half()
-                num += float(np.sum(np.abs(ve - vt) ** 2))
-                den += float(np.sum(np.abs(vt) ** 2))
-            res[k] = np.sqrt(num / max(den, 1e-300))
+            v = meas.forward(skies[k])
+            x = grid_to_healpix(estimates[k], idx, npix)
+            if refine_iters:
+                x, _ = refine(meas, x, v, support, refine_iters, 0.5 * threshold)
+                out[k] = healpix_to_grid(x, idx)
+            ve = meas.forward(x)
+            r = [a - b for a, b in zip(ve, v)]
+            res[k] = np.sqrt(_dot(r, r) / max(_dot(v, v), 1e-300))
     ok = res <= threshold
-    gated = np.where(ok[:, None, None], estimates, dirty)
+    gated = np.where(ok[:, None, None], out, fallback)
     return res, ok, gated
 
 
 def run(pairs=20000, heldout=512, nside=64, img_n=256, batch=16, epochs=2.0, lr=1e-3, fp16=True, seed=1,
-        gate_samples=64, log=None):
+        gate_samples=64, refine_iters=50, log=None):
     """the full experiment; returns a dict of figures"""
     t0 = time.perf_counter()
     seeds = np.arange(100000, 100000 + pairs)
@@ -282,23 +352,49 @@ def run(pairs=20000, heldout=512, nside=64, img_n=256, batch=16, epochs=2.0, lr=
         est = denormalise(net.predict(xh), sh, gain)
     e_net = disk_rel_err(est, Yh)
     e_dirty = disk_rel_err(best_scaled(Xh, Yh), Yh)
+    # the dirty image as inference sees it: its own RMS scale and the training gain
+    dirty_inf = denormalise(xh, sh, gain)
+    e_dirty_inf = disk_rel_err(dirty_inf, Yh)
     cfg = geometry_config(nside, img_n)
     ng = min(gate_samples, heldout)
     with Pipeline(cfg) as p:
         n_dir = p.lmn()[2]
         skies = np.stack([front_sky(int(s), p.plan.npix, n_dir) for s in hseeds[:ng]])
-    res, ok, gated = resimulate_gate(est[:ng], best_scaled(Xh[:ng], Yh[:ng]), skies, cfg)
+    # floor of the grid representation: the true targets put back on the sphere
+    res_floor, _, _ = resimulate_gate(Yh[:ng], dirty_inf[:ng], skies, cfg)
+    # the gate on the network's estimates as they are, then after refinement
+    res_raw, ok_raw, _ = resimulate_gate(est[:ng], dirty_inf[:ng], skies, cfg)
+    res, ok, gated = resimulate_gate(est[:ng], dirty_inf[:ng], skies, cfg, refine_iters=refine_iters)
+    # the same refinement without the network: from the dirty image, and from zero
+    res_d, ok_d, gated_d = resimulate_gate(dirty_inf[:ng], dirty_inf[:ng], skies, cfg, refine_iters=refine_iters)
+    res_z, ok_z, gated_z = resimulate_gate(np.zeros_like(est[:ng]), dirty_inf[:ng], skies, cfg,
+                                           refine_iters=refine_iters)
+
+    def stat(e):
+        return {"mean": float(np.mean(e)), "median": float(np.median(e))}
+
+    e_g = disk_rel_err(gated, Yh[:ng])
     return {
         "pairs": pairs, "heldout": heldout, "nside": nside, "img_n": img_n, "batch": batch, "steps": steps,
         "activations": "fp16" if fp16 else "fp32", "lr": lr,
         "data_s": round(t_data, 2), "train_ms": round(train_ms, 1),
         "train_images_per_s": round(steps * batch / (train_ms * 1e-3), 1),
         "loss_first": float(losses[: max(1, steps // 50)].mean()), "loss_last": float(losses[-max(1, steps // 50):].mean()),
-        "heldout_rel_err_network": {"mean": float(e_net.mean()), "median": float(np.median(e_net))},
-        "heldout_rel_err_dirty_best_gain": {"mean": float(e_dirty.mean()), "median": float(np.median(e_dirty))},
+        "heldout_rel_err_network": stat(e_net),
+        "heldout_rel_err_dirty_best_gain": stat(e_dirty),
+        "heldout_rel_err_dirty_training_gain": stat(e_dirty_inf),
         "network_better_fraction": float(np.mean(e_net < e_dirty)),
-        "gate": {"threshold": GATE_THRESHOLD, "samples": ng, "accepted": int(ok.sum()),
-                 "residual_median": float(np.median(res)), "residual_min": float(res.min())},
+        "gate": {"threshold": GATE_THRESHOLD, "samples": ng, "refine_iters": refine_iters,
+                 "floor_residual_targets": stat(res_floor),
+                 "network_unrefined": {"accepted": int(ok_raw.sum()), "residual": stat(res_raw)},
+                 "network_refined": {"accepted": int(ok.sum()), "residual": stat(res),
+                                     "gated_rel_err": stat(e_g)},
+                 "dirty_refined": {"accepted": int(ok_d.sum()), "residual": stat(res_d),
+                                   "gated_rel_err": stat(disk_rel_err(gated_d, Yh[:ng]))},
+                 "zero_refined": {"accepted": int(ok_z.sum()), "residual": stat(res_z),
+                                  "gated_rel_err": stat(disk_rel_err(gated_z, Yh[:ng]))},
+                 "network_refined_better_fraction_vs_dirty_refined": float(
+                     np.mean(e_g < disk_rel_err(gated_d, Yh[:ng])))},
     }
 
 
@@ -313,8 +409,11 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--fp32", action="store_true", help="FP32 activations (default FP16)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--gate-samples", type=int, default=64)
+    ap.add_argument("--refine-iters", type=int, default=50, help="CGLS iterations of the data-consistency refinement")
     a = ap.parse_args(argv)
     out = run(a.pairs, a.heldout, a.nside, a.img_n, a.batch, a.epochs, a.lr, not a.fp32, a.seed,
+              gate_samples=a.gate_samples, refine_iters=a.refine_iters,
               log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 

@@ -252,6 +252,14 @@ class Pipeline(Stages):
         self._check(self._lib.vw_viss_half(self._ctx, _ptr(out.view(np.float64))))
         return out
 
+    def viss_adjoint(self, res):
+        """adjoint of the current segment's half-visibility sum applied to
+        residual rows res [n_half] (raw convention): [npix] pixel values"""
+        res = np.ascontiguousarray(res, np.complex128)
+        g = np.zeros(self.plan.npix)
+        self._check(self._lib.vw_viss_adjoint(self._ctx, _ptr(res.view(np.float64)), _ptr(g)))
+        return g
+
     def viss_counts(self):
         out = np.zeros(self.nh, np.int64)
         self._check(self._lib.vw_get_viss_counts(self._ctx, _ptr(out, C.c_int64)))
