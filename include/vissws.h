@@ -116,6 +116,7 @@ typedef struct {
   double bl_max_m;
   double cosphi;       /* global threshold (lunar) or NaN (per-antenna) */
   double pix_area;     /* 4 pi / npix                                   */
+  int64_t device_bytes; /* device memory held by the context (0 in the oracle) */
 } vw_plan;
 
 /* Per-segment scalars (printed per segment by main_recon_online.cu:482-494). */

@@ -381,7 +381,7 @@ int vo_get_plan(void* ctx, vw_plan* o) {
   o->npix = c->npix; o->ntile = c->ntile; o->n_img_pts = c->ngp; o->n_img_tiles = c->ngt;
   o->n_pairs = c->P; o->n_epochs = c->T; o->segs = c->segs; o->max_tlen = c->max_tlen;
   o->orbit_res = c->orbit_res; o->seg_len = c->seg_len; o->lambda_m = c->lambda;
-  o->bl_max_m = c->bl_max; o->cosphi = c->cosphi_glob; o->pix_area = c->pix_area;
+  o->bl_max_m = c->bl_max; o->cosphi = c->cosphi_glob; o->pix_area = c->pix_area; o->device_bytes = 0;
   return VW_OK;
 }
 

@@ -13,8 +13,12 @@ Workloads:
   * 64 skies (log-normal + point sources) at Nside 128 on an uneven schedule
     of segments drawn from [16, 256] epochs: the batched form against the
     one-sky plain form on four skies and against the oracle on 1,024 rows;
+    and with every sky on its own such schedule (per-sky weights);
   * the deconvolver at 256 x 256 in FP16: forward and gradients against the
-    oracle with the FP16 storage of the kernels.
+    oracle with the FP16 storage of the kernels;
+  * the reference's lunar formation at 10 MHz and Nside 512: one segment of
+    195,468 half rows in two tile passes, classes bit-exact over every
+    (row, tile), Viss and recon on bounded samples.
 Tolerances as in _lockstep: plain 1e-9, mixed 1e-4 (max |x - ref| / max |ref|
 per output); masks, classes, keys, groups and weights bit-exact.
 """
@@ -198,6 +202,34 @@ def test_nside128_64_skies_uneven_schedule():
     print(f"64 skies, schedule {sched}: Viss vs oracle max {max(verr):.2e}; maps vs plain (C, image) {errs}")
 
 
+def test_nside128_64_skies_per_sky_schedules():
+    """64 skies at Nside 128, each on its own uneven schedule (segments
+    uniform in [16, 256] epochs, one seeded draw per sky): per-sky weights
+    bit-exact and maps within 1e-4 of the one-sky plain FP64 path run on that
+    sky's schedule, for skies 0, 1, 31 and 63"""
+    W = dict(NSIDE128)
+    W.pop("segs")
+    nsky = 64
+    scheds = [_uneven_schedule(W["n_epochs"], 4242 + s) for s in range(nsky)]
+    npix = 12 * 128 ** 2
+    skies = np.stack([make_sky(7000 + s, npix, 1) for s in range(nsky)])
+    with Pipeline(config(precision=1, nsky=nsky, seg_tlen=[256, 256], **W)) as b:
+        b.set_sky(skies)
+        b.set_sky_schedules(scheds)
+        b.run_all()
+        Cb, Wb, Ib, WIb = b.finalize()
+    errs = []
+    for s in (0, 1, 31, 63):
+        with Pipeline(config(seg_tlen=scheds[s], **W)) as p:
+            p.set_sky(skies[s])
+            p.run_all()
+            Cp, Wp, Ip, WIp = p.finalize()
+        assert np.array_equal(Wb[s], Wp) and np.array_equal(WIb[s], WIp), s
+        errs.append((rel_err(Cb[s], Cp), rel_err(Ib[s], Ip)))
+        assert max(errs[-1]) <= MIXED_TOL, (s, errs[-1])
+    print(f"64 skies on their own schedules: maps vs plain (C, image) {errs}")
+
+
 def _unet_params(seed):
     th = U.init_params(seed)
     rng = np.random.default_rng(seed + 1)
@@ -241,3 +273,47 @@ def test_unet_256px_fp16_vs_oracle():
         off += m
     print(f"U-Net 256^2 FP16: output {eo:.2e}, loss {el:.2e}, per-layer gradient max {max(worst):.2e} {worst}")
     assert max(worst) <= UNET_FP16_TOL, worst
+
+
+def test_lunar_nside512_segment_sample():
+    """the reference formation at 10 MHz and Nside 512 (3,145,728 pixels,
+    12,288 tiles), its first segment of 195,468 half rows, whose (tile, row)
+    class tables (2.4 GB) exceed the 2 GiB budget and run in two tile passes:
+    every class bit-exact, masks of 16 rows bit-exact, keys, groups and
+    representatives bit-exact, the plain Viss of 256 rows (first and last
+    128) and the recon of 64 random tiles (16,384 pixels) within 1e-9 of the
+    oracle, their weights bit-exact"""
+    cfg = config(geometry=0, nside=512, freq_hz=1e7, segs=10, img_n=0)
+    rng = np.random.default_rng(9)
+    sky = _sky(12 * 512 ** 2, 19)
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        g.set_sky(sky)
+        o.set_sky(sky)
+        t0, tl = g.schedule()[0]
+        g.gen_segment(t0, tl)
+        o.gen_segment(t0, tl)
+        nh = g.nh
+        assert nh == 195468
+        assert np.array_equal(g.tile_classes(), o.tile_classes())
+        r16 = int(rng.integers(0, nh - 16))
+        assert np.array_equal(g.masks(r16, 16), o.masks(r16, 16))
+        vg = g.viss_half()
+        rows = ((0, 128), (nh - 128, 128))
+        vo = np.concatenate([o.viss_half(r0, nr) for r0, nr in rows])
+        sel = np.concatenate([np.arange(r0, r0 + nr) for r0, nr in rows])
+        ev = rel_err(vg[sel], vo)
+        assert ev <= PLAIN_TOL, ev
+        o.set_viss_raw(vg)
+        hg, _ = g.conj_fill()
+        ho, _ = o.conj_fill()
+        assert rel_err(hg, ho) <= PLAIN_TOL
+        o.group_segment()
+        g.recon_segment()
+        _same_groups(g, o)
+        Cg, Wg, _, _, _ = g.accumulators()
+        pts = _tile_points(g.plan.ntile, g.plan.npix, cfg.tile_pix, 64, rng)
+        cs, ws = o.recon_points(0, pts)
+        assert np.array_equal(Wg[pts], ws)
+        er = rel_err(Cg[pts], cs)
+        assert er <= PLAIN_TOL, er
+    print(f"lunar Nside 512, segment 1: Viss (256 rows) {ev:.2e}, recon (16,384 pixels) {er:.2e}")

@@ -77,6 +77,7 @@ class vw_plan(C.Structure):
         ("bl_max_m", C.c_double),
         ("cosphi", C.c_double),
         ("pix_area", C.c_double),
+        ("device_bytes", C.c_int64),
     ]
 
 

@@ -62,6 +62,7 @@ struct Ctx {
   hipEvent_t ev_recon[2] = {nullptr, nullptr}; /* recon done with segment buffer set b */
   int cur = 1;                                  /* buffer set of the current segment   */
   std::vector<void*> allocs;
+  int64_t dev_bytes = 0; /* bytes of the allocations in allocs */
   double *d_l = nullptr, *d_m = nullptr, *d_n = nullptr, *d_nm1 = nullptr, *d_cone = nullptr;
   double* d_B = nullptr;
   float* d_Bf = nullptr;
@@ -186,6 +187,7 @@ int dalloc(Ctx* c, T** p, size_t n) {
   void* q = nullptr;
   HC(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
   c->allocs.push_back(q);
+  c->dev_bytes += (int64_t)((n ? n : 1) * sizeof(T));
   *p = (T*)q;
   return VW_OK;
 }
@@ -1426,6 +1428,7 @@ VW_API int vw_get_plan(void* p, vw_plan* o) {
   o->bl_max_m = c->bl_max;
   o->cosphi = c->cosphi_glob;
   o->pix_area = c->pix_area;
+  o->device_bytes = c->dev_bytes;
   return VW_OK;
 }
 

@@ -274,7 +274,7 @@ int main(int argc, char** argv) {
   std::printf("Segment plan: segs=%d max_tlen=%d max_segN=%lld max_segN_half=%lld\n", plan.segs, plan.max_tlen,
               (long long)plan.max_tlen * 2 * plan.n_pairs, (long long)plan.max_tlen * plan.n_pairs);
   if (vw_set_sky(ctx, B.data(), npix, 1) != VW_OK) return die(ctx, "vw_set_sky");
-  std::printf("Init GPU in %g s\n", t_init.toc());
+  std::printf("Init GPU in %g s (device memory of the context %.3f GB)\n", t_init.toc(), plan.device_bytes / 1e9);
 
   std::vector<int32_t> seg_t0(plan.segs), seg_tlen(plan.segs);
   vw_get_schedule(ctx, seg_t0.data(), seg_tlen.data());
