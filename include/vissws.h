@@ -319,7 +319,10 @@ int vw_get_groups(void* ctx, vw_seg_info* info, int64_t* keys, int32_t* offsets,
 /* ---------------- host utilities (no device needed) ---------------- */
 
 /* Seeded synthetic skies: kind 0 = log-normal(0,1); kind 1 = log-normal plus
- * Poisson(100) point sources (capped at 200) with Pareto(1.5) flux. */
+ * Poisson(100) point sources (capped at 200) with Pareto(1.5) flux; kind 2 =
+ * a smooth band sky, (1 + 9 exp(-|lat| / 8 deg)) times a log-normal field
+ * exp(0.5 g - 1/8) drawn per Nside-16 pixel (npix = 12 Nside^2, Nside a
+ * power of two >= 16). */
 int vw_make_sky(uint64_t seed, int64_t npix, int32_t kind, double* out);
 
 /* Reference file formats (common.hpp:89-153, main_recon_online.cu:238-267,

@@ -16,6 +16,7 @@ import pytest
 
 import viss_optimize_ws_amd as V
 from viss_optimize_ws_amd import _abi as A
+from viss_optimize_ws_amd import quality
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vissws.h")
@@ -98,6 +99,27 @@ def test_make_sky():
     nsrc = np.count_nonzero(d)
     assert 50 <= nsrc <= 200
     assert d.min() >= 0 and d[d > 0].min() >= 1.0 - 1e-12  # Pareto x_m = 1
+
+
+def test_make_sky_smooth_band():
+    """kind 2: (1 + 9 exp(-|lat| / 8 deg)) times a field that is constant
+    over the NESTED children of each Nside-16 pixel and log-normal(-1/8,
+    1/2) across those pixels"""
+    nside = 64
+    npix = 12 * nside * nside
+    s = V.make_sky(3, npix, 2)
+    assert np.array_equal(s, V.make_sky(3, npix, 2))
+    th, _ = quality.pix2ang_nest(nside, np.arange(npix))
+    lat = np.pi / 2 - th
+    profile = 1.0 + 9.0 * np.exp(-np.abs(lat) * 180.0 / (np.pi * 8.0))
+    field = (s / profile).reshape(12 * 16 * 16, (nside // 16) ** 2)
+    assert np.allclose(field, field[:, :1], rtol=1e-12)  # constant within each Nside-16 pixel
+    g = np.log(field[:, 0])
+    assert abs(g.mean() + 0.125) < 0.05 and abs(g.std() - 0.5) < 0.05
+    with pytest.raises(V.VissError):
+        V.make_sky(3, 12 * 8 * 8, 2)  # below Nside 16
+    with pytest.raises(V.VissError):
+        V.make_sky(3, 1000, 2)  # not a HEALPix size
 
 
 def test_map_and_viss_writers(tmp_path):

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/vissws.h"
+#include "vw_shared.h"
 
 #define VW_API extern "C" __attribute__((visibility("default")))
 
@@ -65,8 +66,32 @@ bool load_txt_auto(const std::string& path, int64_t n, double* out) {
 }  // namespace
 
 VW_API int vw_make_sky(uint64_t seed, int64_t npix, int32_t kind, double* out) {
-  if (!out || npix <= 0 || (kind != 0 && kind != 1)) return VW_ERR_ARG;
+  if (!out || npix <= 0 || kind < 0 || kind > 2) return VW_ERR_ARG;
   std::mt19937_64 g(seed);
+  if (kind == 2) {
+    /* smooth band sky: (1 + 9 exp(-|lat| / 8 deg)) times a log-normal
+     * large-scale field exp(0.5 g - 0.125), g ~ N(0, 1) drawn per Nside-16
+     * pixel (the NESTED parent of each pixel); lat = asin(z) of the pixel
+     * centre.  Needs npix = 12 Nside^2 with Nside a power of two >= 16. */
+    int64_t nside = 1;
+    while (12 * nside * nside < npix) nside *= 2;
+    if (12 * nside * nside != npix || nside < 16) return VW_ERR_ARG;
+    const int64_t ncoarse = 12 * 16 * 16, per = (nside / 16) * (nside / 16);
+    std::vector<double> field((size_t)ncoarse);
+    for (int64_t q = 0; q < ncoarse; q += 2) {
+      double r = std::sqrt(-2.0 * std::log(u01_open0(g)));
+      double th = 2.0 * M_PI * u01(g);
+      field[(size_t)q] = std::exp(0.5 * r * std::cos(th) - 0.125);
+      field[(size_t)q + 1] = std::exp(0.5 * r * std::sin(th) - 0.125);
+    }
+    const double scale = 180.0 / (M_PI * 8.0);
+    for (int64_t p = 0; p < npix; ++p) {
+      double l, m, z;
+      vw_pix2lmn_nest(nside, p, &l, &m, &z);
+      out[p] = (1.0 + 9.0 * std::exp(-std::fabs(std::asin(z)) * scale)) * field[(size_t)(p / per)];
+    }
+    return VW_OK;
+  }
   /* log-normal(0,1) diffuse field, Box-Muller pairs */
   for (int64_t p = 0; p < npix; p += 2) {
     double r = std::sqrt(-2.0 * std::log(u01_open0(g)));
