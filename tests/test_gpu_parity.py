@@ -185,6 +185,37 @@ def test_uneven_segments_and_row_padding(recon_mode, precision):
     assert rel_err(Cg, Co) <= tol and rel_err(Ig, Io) <= tol
 
 
+@pytest.mark.parametrize("form,precision", [(1, 0), (2, 0), (0, 1)], ids=["rows_fp64", "factorized_fp64", "mixed"])
+def test_segments_off_schedule(form, precision):
+    # vw_gen_segment accepts any segment up to the schedule's longest; shorter
+    # ones ask for more tile chunks than the partial buffers were sized for
+    # (sized on the schedule's 64-epoch segments: 35 chunks of the per-row
+    # form, 128 x 64 epochs of the factorized one) and must run with the
+    # chunk count capped.  8 epochs wants 192 per-row chunks; 50 epochs wants
+    # 164 x 50 > 128 x 64 factorized chunk-epochs.
+    cfg = config(nside=64, n_ant=16, n_epochs=128, segs=2, img_n=48, viss_form=form, precision=precision,
+                 freq_hz=2e7)
+    tol = MIXED_TOL if precision else (PLAIN_TOL if form == 1 else ANT_PLAIN_TOL)
+    sky = _sky(12 * 64 * 64, seed=31)
+    with Pipeline(cfg) as g, Oracle(cfg) as o:
+        assert [tl for _, tl in g.schedule()] == [64, 64]
+        g.set_sky(sky)
+        o.set_sky(sky)
+        for t0, tl in ((3, 8), (70, 50), (127, 1)):
+            g.gen_segment(t0, tl)
+            o.gen_segment(t0, tl)
+            err = rel_err(g.viss_half(), o.viss_half())
+            assert err <= tol, (t0, tl, err)
+            g.conj_fill()
+            o.conj_fill()
+            ig, io = g.recon_segment(), o.recon_segment()
+            assert ig["nkeys"] == io["nkeys"]
+        Cg, Wg, Ig, WIg = g.finalize()
+        Co, Wo, Io, WIo = o.finalize()
+    assert np.array_equal(Wg, Wo) and np.array_equal(WIg, WIo)
+    assert rel_err(Cg, Co) <= tol and rel_err(Ig, Io) <= tol
+
+
 def test_lunar_30mhz_band():
     # the reference formation at its 30 MHz band (bl_max 100 km, RES ~ 46,000)
     cfg = config(geometry=0, nside=8, freq_hz=3e7, segs=20, img_n=32)

@@ -401,7 +401,12 @@ constexpr int viss_rpw(int /*TP*/, bool mixed, bool count) {
   return (mixed ? VW_VISS_RPW_MIXED : VW_VISS_RPW_PLAIN) / (count ? 2 : 1);
 }
 
-VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {
+/* The launch shapes take lim, the chunk count the context's partial buffers
+ * hold (0 while they are being sized): a segment shorter than the schedule's
+ * would ask for more chunks than the longest one, and gets lim chunks of more
+ * tiles each instead.  lim is at least the chunk count of a full tile pass of
+ * the longest segment, so the tiles-per-chunk caps below still hold. */
+VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false, int64_t lim = 0) {
   VissShape s;
   bool mixed = c->cfg.precision == VW_PREC_MIXED;
   s.rpw = viss_rpw(c->TP, mixed, count);
@@ -409,6 +414,7 @@ VissShape viss_shape(const Ctx* c, int64_t nh, int64_t nt, bool count = false) {
   const int64_t target_waves = VW_VISS_TARGET_WAVES;
   int64_t nchunk = (target_waves + s.nrb - 1) / s.nrb;
   nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, nt));
+  if (lim > 0) nchunk = std::min(nchunk, lim);
   s.tpc = (nt + nchunk - 1) / nchunk;
   const int64_t tpc_cap = mixed ? 16 : 64; /* bounds the FP32 block length per lane */
   if (s.tpc > tpc_cap) s.tpc = tpc_cap;
@@ -428,10 +434,11 @@ struct VissFShape {
 };
 inline int vissf_groups(int A) { return A <= 16 ? 1 : 2; }
 inline int64_t vissf_blocks(int G) { return (int64_t)G * (G + 1) / 2 + (int64_t)G * G; }
-VissFShape viss_fshape(const Ctx* c, int64_t tlen, int64_t nt) {
+VissFShape viss_fshape(const Ctx* c, int64_t tlen, int64_t nt, int64_t lim = 0) {
   VissFShape s;
   int64_t nchunk = (VW_VISSF_TARGET_WAVES + tlen - 1) / tlen;
   nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, nt));
+  if (lim > 0) nchunk = std::min(nchunk, lim);
   s.tpc = (nt + nchunk - 1) / nchunk;
   s.nchunk = (nt + s.tpc - 1) / s.tpc;
   s.nquad = (tlen + 3) / 4;
@@ -467,12 +474,13 @@ constexpr int64_t VW_BATCH_TARGETS[] = {2048, 4096, 8192, 16384, 32768, 65536};
 struct VissBShape {
   int64_t nrb, nrbw, tpc, nchunk, nblocks;
 };
-VissBShape vissb_shape(const Ctx* c, int64_t nh, int64_t nt, int64_t target) {
+VissBShape vissb_shape(const Ctx* c, int64_t nh, int64_t nt, int64_t target, int64_t lim = 0) {
   VissBShape s;
   s.nrb = (nh + 15) / 16;
   s.nrbw = (s.nrb + 3) / 4;
   int64_t nchunk = (target + s.nrb * c->nsg - 1) / (s.nrb * c->nsg);
   nchunk = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nchunk, nt), VW_BATCH_MAX_CHUNKS));
+  if (lim > 0) nchunk = std::min(nchunk, lim);
   s.tpc = (nt + nchunk - 1) / nchunk;
   s.nchunk = (nt + s.tpc - 1) / s.tpc;
   s.nblocks = s.nchunk * s.nrbw * c->nsg;
@@ -828,7 +836,7 @@ void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
 int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
   const bool mixed = c->cfg.precision == VW_PREC_MIXED;
   const int G = vissf_groups(c->A);
-  VissFShape s = viss_fshape(c, c->tlen, nt);
+  VissFShape s = viss_fshape(c, c->tlen, nt, c->fpart_cap / ((int64_t)c->tlen * vissf_blocks(G) * 256));
   if ((int64_t)c->tlen * s.nchunk * vissf_blocks(G) * 256 > c->fpart_cap)
     return c->fail(VW_ERR_RANGE, "internal: factorized partial buffer too small");
   const int64_t nfr = (int64_t)c->tlen * c->A;
@@ -916,7 +924,7 @@ int enq_order(Ctx* c, int which) {
 int enq_viss_batch(Ctx* c, int64_t tile0, int64_t nt) {
   int rc;
   if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
-  const VissBShape s = vissb_shape(c, c->nh, nt, c->vb_target);
+  const VissBShape s = vissb_shape(c, c->nh, nt, c->vb_target, c->bpart_cap / (c->nsky_pad * c->nh_pad));
   if (s.nchunk * c->nsky_pad * c->nh_pad > c->bpart_cap) return c->fail(VW_ERR_RANGE, "internal: batch partial buffer too small");
   VissBArgs a{};
   a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1;
@@ -965,7 +973,7 @@ int enq_viss(Ctx* c, bool count) {
       if ((rc = enq_viss_fact(c, tile0, nt))) return rc;
       continue;
     }
-    VissShape s = viss_shape(c, c->nh, nt, count);
+    VissShape s = viss_shape(c, c->nh, nt, count, c->nchunk_max);
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
     if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
     VissArgs a{};
