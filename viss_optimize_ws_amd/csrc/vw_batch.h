@@ -215,16 +215,23 @@ __device__ __forceinline__ vw_bf16x8 vw_lds_bf16x8(const __bf16* buf, int row, i
 
 /* one wave: 16 half rows x (16 NB) skies over the tiles of one chunk; a
  * workgroup is four waves (64 consecutive rows) on the same pixels.  Per
- * k-step of 32 pixels the workgroup stages the sky operand (and, per
- * 256-pixel sub-tile, l, m, n - 1, n) through double-buffered LDS. */
+ * k-step of 32 pixels the workgroup stages the sky operand, the pixels' l,
+ * m, n - 1, n and the 64 rows' class bytes on the step's tile through
+ * double-buffered LDS.  Every step issues the same global loads, for the
+ * next step, unconditionally (indices clamped in range), and waits for them
+ * only before their LDS stores at its end: a load waited for right after it
+ * is issued would also wait for the loads in flight before it (the vector
+ * memory counter is in order), exposing their latency. */
 #ifndef VW_VISSB_WAVES
-#define VW_VISSB_WAVES 2
+#define VW_VISSB_WAVES 3
 #endif
 template <int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WAVES))) void k_vissb(VissBArgs a) {
   using Stage = VwStage<2, NB>;
   __shared__ __attribute__((aligned(16))) __bf16 bsh[2][Stage::ROWS * VW_LROW];
-  __shared__ double px[2][4][256]; /* l, m, n - 1, n of a sub-tile */
+  __shared__ double px[2][4][32]; /* l, m, n - 1, n of a k-step's pixels */
+  __shared__ uint8_t pcls[2][64]; /* class bytes of the 64 rows on a k-step's tile */
+  __shared__ double frs[10][64];  /* the 64 rows' antenna frames (mixed-tile masks) */
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t rbw = vb % a.nrbw;
   const int64_t rest = vb / a.nrbw;
@@ -239,84 +246,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
   const int64_t h = jr < a.nh ? a.perm[jr] : jr;
   const int64_t hc = h < a.nh ? h : a.nh - 1;
   const RowRec rec = a.rows[hc];
+  /* the rows' antenna frames (per-pixel masks on mixed tiles), held in LDS
+   * rather than in 20 registers per lane: read only on mixed tiles */
+  if (g == 0) {
+    const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+    const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+    const int w_ = wave * 16 + r;
+    frs[0][w_] = qa[0]; frs[1][w_] = qa[1]; frs[2][w_] = qa[2]; frs[3][w_] = qa[3]; frs[4][w_] = qa[7];
+    frs[5][w_] = qb[0]; frs[6][w_] = qb[1]; frs[7][w_] = qb[2]; frs[8][w_] = qb[3]; frs[9][w_] = qb[7];
+  }
   const int64_t t0 = a.tile0 + chunk * a.tpc;
   const int64_t tend = a.tile0 + a.ntile;
   const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
-  const int64_t nsub = a.TP / 256;
-  const int64_t st0 = t0 * nsub;
-  const int64_t K = (t1 - t0) * nsub * 8; /* k-steps of 32 pixels */
+  const int ksh = a.TP == 512 ? 4 : 3; /* k-steps per tile: 1 << ksh */
+  const int64_t kb0 = t0 << ksh;       /* first k-step block of the chunk */
+  const int64_t K = (t1 - t0) << ksh;  /* k-steps of 32 pixels (> 0) */
   const int64_t row0 = sg * 16 * NB;
 
   Stage stg;
-  double pr0 = 0.0, pr1 = 0.0, pr2 = 0.0, pr3 = 0.0; /* this thread's pixel of the next sub-tile */
-#define VW_P_LOAD(st_)                                 \
-  do {                                                 \
-    const int64_t p_ = (st_) * 256 + threadIdx.x;      \
-    const bool ok_ = p_ < a.npix;                      \
-    const int64_t q_ = ok_ ? p_ : 0;                   \
-    pr0 = ok_ ? a.l[q_] : 0.0;                         \
-    pr1 = ok_ ? a.m[q_] : 0.0;                         \
-    pr2 = ok_ ? a.nm1[q_] : 0.0;                       \
-    pr3 = ok_ ? a.n[q_] : 0.0;                         \
+  /* thread t: coordinate (t / 32) % 4 of pixel t % 32 of a step (threads
+   * 128-255 repeat 0-127, only the first store); lane (r, g): its row's
+   * class byte (only g = 0 stores) */
+  const int xc_ = (int)((threadIdx.x >> 5) & 3);
+  const double* xsrc = xc_ == 0 ? a.l : (xc_ == 1 ? a.m : (xc_ == 2 ? a.nm1 : a.n));
+  double xp;
+  int xcl;
+#define VW_X_LOAD(k_)                                                                   \
+  do {                                                                                  \
+    const int64_t p_ = ((kb0 + (k_)) << 5) + (threadIdx.x & 31);                        \
+    const double v_ = xsrc[p_ < a.npix ? p_ : a.npix - 1];                              \
+    xp = p_ < a.npix ? v_ : 0.0;                                                        \
+    /* rows past nh carry the hidden class in the table */                             \
+    const int c_ = (int)a.cls[(((kb0 + (k_)) >> ksh) - a.tile0) * a.nh_pad + h];       \
+    xcl = wave_on ? c_ : VW_CLS_HIDDEN;                                                 \
   } while (0)
-#define VW_P_STORE(b_)                                                        \
-  do {                                                                        \
-    px[b_][0][threadIdx.x] = pr0; px[b_][1][threadIdx.x] = pr1;               \
-    px[b_][2][threadIdx.x] = pr2; px[b_][3][threadIdx.x] = pr3;               \
+#define VW_X_STORE(b_)                                           \
+  do {                                                           \
+    if (threadIdx.x < 128) px[b_][xc_][threadIdx.x & 31] = xp;   \
+    if (g == 0) pcls[b_][wave * 16 + r] = (uint8_t)xcl;          \
   } while (0)
 
-  vw_f32x4b accR[NB], accI[NB];
-  double sumR[NB][4], sumI[NB][4];
+  vw_f32x4b accR[NB], accI[NB], sumR[NB], sumI[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
     accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) { sumR[nb][i] = 0.0; sumI[nb][i] = 0.0; }
+    sumR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
+    sumI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
   }
 
-  /* prologue: step 0 and sub-tile 0 */
-  VW_P_LOAD(st0);
-  stg.load(a.skyh, a.skyl, nullptr, nullptr, st0 * 8, a.nsky_pad, row0);
-  VW_P_STORE(0);
+  /* prologue: step 0 */
+  VW_X_LOAD(0);
+  stg.load(a.skyh, a.skyl, nullptr, nullptr, kb0, a.nsky_pad, row0);
+  VW_X_STORE(0);
   stg.store(bsh[0]);
   __syncthreads();
 
-  int cls = VW_CLS_HIDDEN;
-  bool act = false, any_mixed = false, frames = false;
-  double ax = 0, ay = 0, az = 0, ai = 0, ac = 0, bx = 0, by = 0, bz = 0, bi = 0, bc = 0;
   for (int64_t k = 0; k < K; ++k) {
-    const int64_t sidx = k >> 3;
-    const int ks = (int)(k & 7);
-    const int pbuf = (int)(sidx & 1), bbuf = (int)(k & 1);
-    const bool more = k + 1 < K;
-    if (more) stg.load(a.skyh, a.skyl, nullptr, nullptr, st0 * 8 + k + 1, a.nsky_pad, row0);
-    if (ks == 0) {
-      /* a new sub-tile: its pixels for the next one, this one's classes */
-      if (sidx + 1 < K / 8) VW_P_LOAD(st0 + sidx + 1);
-      const int64_t tile = (st0 + sidx) / nsub;
-      /* rows past nh carry the hidden class in the table */
-      cls = wave_on ? (int)a.cls[(tile - a.tile0) * a.nh_pad + h] : VW_CLS_HIDDEN;
-      act = __ballot(cls != VW_CLS_HIDDEN) != 0;
-      any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
-      if (any_mixed && cls == VW_CLS_MIXED && !frames) {
-        /* the row's antenna frames: loaded once, at its first mixed sub-tile */
-        const double* qa = a.ant8 + (int64_t)rec.ia * 8;
-        const double* qb = a.ant8 + (int64_t)rec.ib * 8;
-        ax = qa[0]; ay = qa[1]; az = qa[2]; ai = qa[3]; ac = qa[7];
-        bx = qb[0]; by = qb[1]; bz = qb[2]; bi = qb[3]; bc = qb[7];
-        frames = true;
-      }
-    }
+    const int buf = (int)(k & 1);
+    const int64_t kn = k + 1 < K ? k + 1 : k; /* the last step reloads itself */
+    stg.load(a.skyh, a.skyl, nullptr, nullptr, kb0 + kn, a.nsky_pad, row0);
+    VW_X_LOAD(kn);
+    const int cls = (int)pcls[buf][wave * 16 + r];
+    const bool act = __ballot(cls != VW_CLS_HIDDEN) != 0;
+    const bool any_mixed = __ballot(cls == VW_CLS_MIXED) != 0;
     if (act) {
-      const double* pl_ = px[pbuf][0];
-      const double* pm_ = px[pbuf][1];
-      const double* pz_ = px[pbuf][2];
-      const double* pn_ = px[pbuf][3];
+      const double* pl_ = px[buf][0];
+      const double* pm_ = px[buf][1];
+      const double* pz_ = px[buf][2];
+      const double* pn_ = px[buf][3];
       float cv[8], sv[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const int kk = ks * 32 + g * 8 + i;
+        const int kk = g * 8 + i;
         const double pl = pl_[kk], pm = pm_[kk], pz = pz_[kk];
         const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl, pm, pz);
         const float f = (float)__builtin_amdgcn_fract(x);
@@ -324,7 +326,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
         bool on = cls == VW_CLS_VISIBLE;
         if (any_mixed && cls == VW_CLS_MIXED) {
           const double pn = pn_[kk];
-          on = vw_visible(pl, pm, pn, ax, ay, az, ai, ac) && vw_visible(pl, pm, pn, bx, by, bz, bi, bc);
+          const int w_ = wave * 16 + r;
+          on = vw_visible(pl, pm, pn, frs[0][w_], frs[1][w_], frs[2][w_], frs[3][w_], frs[4][w_]) &&
+               vw_visible(pl, pm, pn, frs[5][w_], frs[6][w_], frs[7][w_], frs[8][w_], frs[9][w_]);
         }
         cv[i] = on ? cs : 0.0f;
         sv[i] = on ? sn : 0.0f;
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
       vw_bf16x8 ach, acl, ash, asl;
       vw_split8(cv, ach, acl);
       vw_split8(sv, ash, asl);
-      const __bf16* bb = bsh[bbuf];
+      const __bf16* bb = bsh[buf];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
         const vw_bf16x8 bh = vw_lds_bf16x8(bb, nb * 16 + r, g);
@@ -340,30 +344,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
         accR[nb] = vw_mac3(ach, acl, bh, bl, accR[nb]);
         accI[nb] = vw_mac3(ash, asl, bh, bl, accI[nb]);
       }
-      if (ks == 7) {
-        /* FP64 sums across sub-tiles */
+      if ((k & 7) == 7) {
+        /* blocked sums: the products of 256 pixels accumulate in the matrix
+         * core's FP32 accumulators, which are then added to FP32 block sums
+         * (folded across chunks in FP64): error ~ (256 + K / 256) ulp
+         * rather than K ulp.  A tile's class is the same on all its steps,
+         * so act holds on the whole block */
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            sumR[nb][i] += (double)accR[nb][i];
-            sumI[nb][i] += (double)accI[nb][i];
-          }
+          sumR[nb] += accR[nb];
+          sumI[nb] += accI[nb];
           accR[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
           accI[nb] = vw_f32x4b{0.f, 0.f, 0.f, 0.f};
         }
       }
     }
-    if (more) {
-      /* buffers of step k - 1 / sub-tile sidx - 1: every wave passed the
-       * barrier after reading them */
-      stg.store(bsh[bbuf ^ 1]);
-      if (ks == 7) VW_P_STORE(pbuf ^ 1);
-    }
+    /* buffers of step k - 1: every wave passed the barrier after reading
+     * them (the last step writes its own reload there, unread) */
+    stg.store(bsh[buf ^ 1]);
+    VW_X_STORE(buf ^ 1);
     __syncthreads();
   }
-#undef VW_P_LOAD
-#undef VW_P_STORE
+#undef VW_X_LOAD
+#undef VW_X_STORE
   if (!wave_on) return;
   /* D[4 g + i][r]: block row rb * 16 + 4 g + i, sky column r */
   int64_t hw[4];
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
     const int64_t sky = (sg * NB + nb) * 16 + r;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (hw[i] >= 0) a.part[(chunk * a.nsky_pad + sky) * a.nh_pad + hw[i]] = make_double2(sumR[nb][i], -sumI[nb][i]);
+      if (hw[i] >= 0) a.part[(chunk * a.nsky_pad + sky) * a.nh_pad + hw[i]] = make_double2((double)sumR[nb][i], -(double)sumI[nb][i]);
   }
 }
 
