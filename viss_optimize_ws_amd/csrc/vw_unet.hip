@@ -116,6 +116,12 @@ __global__ __launch_bounds__(256) void k_conv3(const T* __restrict__ in, const f
  * tap, 8 consecutive channels) are one 16-byte read. */
 typedef _Float16 vw_h8 __attribute__((ext_vector_type(8)));
 typedef float vw_f4u __attribute__((ext_vector_type(4)));
+/* logical workgroup index with consecutive indices on one XCD (workgroups
+ * are dispatched to the 8 XCDs round-robin) */
+__device__ __forceinline__ int64_t vu_xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
 typedef _Float16 vw_h2 __attribute__((ext_vector_type(2)));
 template <bool RELU, int CB, bool PF, bool VS>
 __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
@@ -125,9 +131,16 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
   constexpr int STEPS = CB == 32 ? 9 : 5;
   __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * 40]; /* [y][x][ci], rows of 32 (+8 pad) */
   const int tpr = (H + 15) / 16;
-  const int ty0 = (blockIdx.x / tpr) * 16, tx0 = (blockIdx.x % tpr) * 16;
-  const int ob = blockIdx.y * 16;
-  const int n = blockIdx.z;
+  /* one-dimensional grid, workgroups ordered (output-channel block, tile x,
+   * tile y, image) from fastest and mapped so that consecutive ones run on
+   * one XCD: the blocks sharing an input tile, and the tiles sharing its
+   * cache lines, meet in one L2 */
+  const int nob = (cout + 15) / 16;
+  const int vb = (int)vu_xcd_block(blockIdx.x, gridDim.x);
+  const int ob = (vb % nob) * 16;
+  const int tl = (vb / nob) % (tpr * tpr);
+  const int n = vb / (nob * tpr * tpr);
+  const int ty0 = (tl / tpr) * 16, tx0 = (tl % tpr) * 16;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, g = lane >> 4;
   const int64_t HH = (int64_t)H * H;
@@ -142,10 +155,8 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
      * for g >= 2); with PF all of them are issued before the staging loads so
      * their latency overlaps it (for launches too small to hide it by
      * occupancy: PF costs registers), without PF one per k-step.  VS stages
-     * the tile with 16-byte loads of channel pairs; where one output-channel
-     * block covers the layer (the input tile is read once) the element-wise
-     * staging measured faster: its load instructions each touch a few cache
-     * lines of one channel. */
+     * the tile with 16-byte loads of channel pairs (the element-wise form is
+     * kept for H % 8 != 0). */
     vw_h8 aw[PF ? STEPS : 1];
     if constexpr (PF) {
 #pragma unroll
@@ -222,7 +233,43 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
     }
 #undef VW_CONV_KSTEP
   }
-  /* D[4 g + i][r]: output channel ob + 4 g + i, pixel (4 wave + j, r) */
+  /* D[4 g + i][r]: output channel ob + 4 g + i, pixel (4 wave + j, r).  With
+   * VS (H % 8 == 0: whole 16-pixel rows) the tile goes through LDS as
+   * [16 channels][16 rows][16 + 8 pixels] halves and leaves as 16-byte
+   * stores of 8 pixels (with 16-byte mask loads), instead of 2-byte stores
+   * of 16 lanes per channel row. */
+  if (VS && H % 8 == 0) {
+    __syncthreads(); /* every wave is done with the input tile */
+    _Float16* ob_ = tile;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int o = ob + 4 * g + i;
+        float v = acc[j][i] + ((B && o < cout) ? B[o] : 0.f);
+        if (RELU) v = v > 0.f ? v : 0.f;
+        ob_[((4 * g + i) * 16 + 4 * wave + j) * 24 + r] = (_Float16)v;
+      }
+    __syncthreads();
+    /* 16 channels x 16 rows x 2 halves of 8 pixels: 512 stores, 2 per thread */
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = (int)threadIdx.x + 256 * q;
+      const int hx = e & 1, yy = (e >> 1) & 15, oc = e >> 5;
+      const int o = ob + oc, y = ty0 + yy, x = tx0 + 8 * hx;
+      if (o >= cout || y >= H || x >= H) continue;
+      vw_h8 v = *(const vw_h8*)(ob_ + (oc * 16 + yy) * 24 + 8 * hx);
+      const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
+      if (mask) {
+        const vw_h8 mk = *(const vw_h8*)(mask + oi);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (!((float)mk[k] > 0.f)) v[k] = (_Float16)0.f;
+      }
+      *(vw_h8*)(out + oi) = v;
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int y = ty0 + 4 * wave + j, x = tx0 + r;
@@ -438,10 +485,16 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
   static_assert(3 * 64 * 37 * 4 <= (VW_WGM_SX + VW_WGM_SG) * 16, "reduction area");
   vw_h8* sx = arena;
   vw_h8* sg = arena + VW_WGM_SX;
-  const int cib = blockIdx.x * 16, cob = blockIdx.y * 16;
+  /* one-dimensional grid ordered (input-channel block, output-channel
+   * block, chunk) from fastest, consecutive workgroups on one XCD: the
+   * blocks of a chunk read the same input and gradient tiles */
+  const int nib = (ci_n + 15) / 16, nob = co_n / 16;
+  const int vb = (int)vu_xcd_block(blockIdx.x, gridDim.x);
+  const int bz = vb / (nib * nob);
+  const int cib = (vb % nib) * 16, cob = ((vb / nib) % nob) * 16;
   const int tpy = (H + 15) / 16, tpx = (H + 31) / 32;
   const int64_t ntiles = (int64_t)N * tpy * tpx;
-  const int64_t t0 = (int64_t)blockIdx.z * tpc;
+  const int64_t t0 = (int64_t)bz * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int r = lane & 15, g = lane >> 4;
@@ -560,7 +613,7 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
     /* D[4 g + i][r]: output channel cob + 4 g + i, input channel cib + r */
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      float* dst = part + (((int64_t)blockIdx.z * co_n + cob + 4 * g + i) * ci_n + cib + r) * 10;
+      float* dst = part + (((int64_t)bz * co_n + cob + 4 * g + i) * ci_n + cib + r) * 10;
       if (cib + r < ci_n) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) dst[k] = acc[k][i];
@@ -569,7 +622,7 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(const _Float16* __rest
     /* the bias of output channel cob + r: the four chunk groups of the lanes */
     bsum += __shfl_xor(bsum, 16);
     bsum += __shfl_xor(bsum, 32);
-    if (cib == 0 && g == 0) part[(((int64_t)blockIdx.z * co_n + cob + r) * ci_n) * 10 + 9] = bsum;
+    if (cib == 0 && g == 0) part[(((int64_t)bz * co_n + cob + r) * ci_n) * 10 + 9] = bsum;
   }
 }
 
@@ -986,16 +1039,15 @@ int ualloc(Net* c, P** p, size_t bytes) {
 
 /* matrix-core convolution launch: 32-channel LDS chunks when cin allows;
  * weight prefetch for launches of fewer than 4,096 workgroups; vector
- * staging with the prefetch or when several output-channel blocks share a
- * tile (choices measured at batch 16, 256 x 256) */
+ * staging (choices measured at batch 16, 256 x 256: with the XCD-grouped
+ * workgroup order the vector staging is faster on every layer) */
 template <bool RELU, int CB>
 void conv3_mfma_cb(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
                    int cout, int H, const _Float16* mask) {
-  const bool pf = (int64_t)grid.x * grid.y * grid.z < 4096;
-  if (pf) k_conv3_mfma<RELU, CB, true, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
-  else if (grid.y >= 2)
-    k_conv3_mfma<RELU, CB, false, true><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
-  else k_conv3_mfma<RELU, CB, false, false><<<grid, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  const unsigned nwg = grid.x * grid.y * grid.z; /* (tiles, output-channel blocks, images) */
+  const bool pf = nwg < 4096;
+  if (pf) k_conv3_mfma<RELU, CB, true, true><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else k_conv3_mfma<RELU, CB, false, true><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
 }
 template <bool RELU>
 void conv3_mfma(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
@@ -1050,8 +1102,8 @@ struct Ops {
     float* part = c->wpart + c->woff[li];
     const dim3 wgrid((unsigned)((L.ci + 15) / 16), (unsigned)((L.co + 15) / 16), (unsigned)wp.nch);
     if (wp.mfma)
-      k_conv3_wgrad_mfma<<<wgrid, 256, 0, c->st>>>((const _Float16*)c->a[src], (const _Float16*)dzp, part, n,
-                                                   L.ci, L.co, H, wp.tpc);
+      k_conv3_wgrad_mfma<<<wgrid.x * wgrid.y * wgrid.z, 256, 0, c->st>>>(
+          (const _Float16*)c->a[src], (const _Float16*)dzp, part, n, L.ci, L.co, H, wp.tpc);
     else
       k_conv3_wgrad_t<T><<<wgrid, 256, 0, c->st>>>((const T*)c->a[src], (const T*)dzp, part, n, L.ci, L.co, H);
     UC(hipGetLastError());
