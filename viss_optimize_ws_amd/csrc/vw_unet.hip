@@ -123,7 +123,7 @@ __device__ __forceinline__ int64_t vu_xcd_block(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + i;
 }
 typedef _Float16 vw_h2 __attribute__((ext_vector_type(2)));
-template <bool RELU, int CB, bool PF, bool VS>
+template <bool RELU, int CB, bool PF, bool VS, int OCB>
 __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__ in, const _Float16* __restrict__ Wk,
                                                     const float* __restrict__ B, _Float16* __restrict__ out, int cin,
                                                     int cout, int H, const _Float16* __restrict__ mask) {
@@ -135,9 +135,9 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
    * tile y, image) from fastest and mapped so that consecutive ones run on
    * one XCD: the blocks sharing an input tile, and the tiles sharing its
    * cache lines, meet in one L2 */
-  const int nob = (cout + 15) / 16;
+  const int nob = (cout + 16 * OCB - 1) / (16 * OCB); /* blocks of 16 OCB output channels */
   const int vb = (int)vu_xcd_block(blockIdx.x, gridDim.x);
-  const int ob = (vb % nob) * 16;
+  const int ob = (vb % nob) * 16 * OCB;
   const int tl = (vb / nob) % (tpr * tpr);
   const int n = vb / (nob * tpr * tpr);
   const int ty0 = (tl / tpr) * 16, tx0 = (tl % tpr) * 16;
@@ -146,9 +146,12 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
   const int64_t HH = (int64_t)H * H;
   const int K = 9 * cin;
   constexpr int cb = CB; /* channels per LDS chunk (a divisor of cin) */
-  vw_f4u acc[4];
+  /* acc[b][j]: output channels ob + 16 b .., pixel row 4 wave + j */
+  vw_f4u acc[OCB][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) acc[j] = vw_f4u{0.f, 0.f, 0.f, 0.f};
+  for (int b = 0; b < OCB; ++b)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[b][j] = vw_f4u{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < cin; c0 += cb) {
     /* the chunk's weight operands (k-steps: (tap, 32 channels) or, with
      * 16-channel chunks, (tap, 16 channels) for lanes g < 2 and the next tap
@@ -215,19 +218,39 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
       const int py = 4 * wave + j; /* tile row of this column block */                          \
       vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * 40 + ci0);                  \
       if (!kon) bv = vw_h8{};                                                                   \
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av_, bv, acc[j], 0, 0, 0);                \
+      _Pragma("unroll") for (int b = 0; b < OCB; ++b)                                            \
+        acc[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av_[b], bv, acc[b][j], 0, 0, 0);      \
     }                                                                                           \
   } while (0)
     if constexpr (PF) {
 #pragma unroll
-      for (int ks = 0; ks < STEPS; ++ks) VW_CONV_KSTEP(aw[ks]);
-    } else {
+      for (int ks = 0; ks < STEPS; ++ks) {
+        const vw_h8 av1[1] = {aw[ks]};
+        VW_CONV_KSTEP(av1);
+      }
+    } else if constexpr (OCB == 1) {
 #pragma unroll 1
       for (int ks = 0; ks < STEPS; ++ks) {
         const int tap = CB == 32 ? ks : 2 * ks + (g >> 1);
         const int ci0 = CB == 32 ? 8 * g : 8 * (g & 1);
-        vw_h8 av = {};
-        if (tap < 9 && ob + r < cout) av = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
+        vw_h8 av[1] = {{}};
+        if (tap < 9 && ob + r < cout) av[0] = *(const vw_h8*)(Wk + ((int64_t)(ob + r) * K + tap * cin + c0 + ci0));
+        VW_CONV_KSTEP(av);
+      }
+    } else {
+      /* OCB weight operands per k-step, one per 16-channel block; unrolled
+       * so that their loads are issued ahead of the products */
+#pragma unroll
+      for (int ks = 0; ks < STEPS; ++ks) {
+        const int tap = CB == 32 ? ks : 2 * ks + (g >> 1);
+        const int ci0 = CB == 32 ? 8 * g : 8 * (g & 1);
+        vw_h8 av[OCB];
+#pragma unroll
+        for (int b = 0; b < OCB; ++b) {
+          av[b] = vw_h8{};
+          const int o = ob + 16 * b + r;
+          if (tap < 9 && o < cout) av[b] = *(const vw_h8*)(Wk + ((int64_t)o * K + tap * cin + c0 + ci0));
+        }
         VW_CONV_KSTEP(av);
       }
     }
@@ -239,52 +262,58 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
    * stores of 8 pixels (with 16-byte mask loads), instead of 2-byte stores
    * of 16 lanes per channel row. */
   if (VS && H % 8 == 0) {
-    __syncthreads(); /* every wave is done with the input tile */
     _Float16* ob_ = tile;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int b = 0; b < OCB; ++b) {
+      const int obb = ob + 16 * b;
+      __syncthreads(); /* every wave is done with the input tile / the last block's stores */
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int o = ob + 4 * g + i;
-        float v = acc[j][i] + ((B && o < cout) ? B[o] : 0.f);
-        if (RELU) v = v > 0.f ? v : 0.f;
-        ob_[((4 * g + i) * 16 + 4 * wave + j) * 24 + r] = (_Float16)v;
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int o = obb + 4 * g + i;
+          float v = acc[b][j][i] + ((B && o < cout) ? B[o] : 0.f);
+          if (RELU) v = v > 0.f ? v : 0.f;
+          ob_[((4 * g + i) * 16 + 4 * wave + j) * 24 + r] = (_Float16)v;
+        }
+      __syncthreads();
+      /* 16 channels x 16 rows x 2 halves of 8 pixels: 512 stores, 2 per thread */
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        const int hx = e & 1, yy = (e >> 1) & 15, oc = e >> 5;
+        const int o = obb + oc, y = ty0 + yy, x = tx0 + 8 * hx;
+        if (o >= cout || y >= H || x >= H) continue;
+        vw_h8 v = *(const vw_h8*)(ob_ + (oc * 16 + yy) * 24 + 8 * hx);
+        const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
+        if (mask) {
+          const vw_h8 mk = *(const vw_h8*)(mask + oi);
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (!((float)mk[k] > 0.f)) v[k] = (_Float16)0.f;
+        }
+        *(vw_h8*)(out + oi) = v;
       }
-    __syncthreads();
-    /* 16 channels x 16 rows x 2 halves of 8 pixels: 512 stores, 2 per thread */
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int e = (int)threadIdx.x + 256 * q;
-      const int hx = e & 1, yy = (e >> 1) & 15, oc = e >> 5;
-      const int o = ob + oc, y = ty0 + yy, x = tx0 + 8 * hx;
-      if (o >= cout || y >= H || x >= H) continue;
-      vw_h8 v = *(const vw_h8*)(ob_ + (oc * 16 + yy) * 24 + 8 * hx);
-      const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
-      if (mask) {
-        const vw_h8 mk = *(const vw_h8*)(mask + oi);
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (!((float)mk[k] > 0.f)) v[k] = (_Float16)0.f;
-      }
-      *(vw_h8*)(out + oi) = v;
     }
     return;
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int y = ty0 + 4 * wave + j, x = tx0 + r;
-    if (y >= H || x >= H) continue;
+  for (int b = 0; b < OCB; ++b)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int o = ob + 4 * g + i;
-      if (o >= cout) continue;
-      float v = acc[j][i] + (B ? B[o] : 0.f);
-      if (RELU) v = v > 0.f ? v : 0.f;
-      const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
-      if (mask && !((float)mask[oi] > 0.f)) v = 0.f;
-      out[oi] = (_Float16)v;
+    for (int j = 0; j < 4; ++j) {
+      const int y = ty0 + 4 * wave + j, x = tx0 + r;
+      if (y >= H || x >= H) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int o = ob + 16 * b + 4 * g + i;
+        if (o >= cout) continue;
+        float v = acc[b][j][i] + (B ? B[o] : 0.f);
+        if (RELU) v = v > 0.f ? v : 0.f;
+        const int64_t oi = ((int64_t)n * cout + o) * HH + (int64_t)y * H + x;
+        if (mask && !((float)mask[oi] > 0.f)) v = 0.f;
+        out[oi] = (_Float16)v;
+      }
     }
-  }
 }
 
 /* FP16 weight layouts for the matrix-core convolution, all 3x3 layers in
@@ -1041,13 +1070,26 @@ int ualloc(Net* c, P** p, size_t bytes) {
  * weight prefetch for launches of fewer than 4,096 workgroups; vector
  * staging (choices measured at batch 16, 256 x 256: with the XCD-grouped
  * workgroup order the vector staging is faster on every layer) */
+#ifndef VW_CONV_MIN_WG
+#define VW_CONV_MIN_WG 512
+#endif
 template <bool RELU, int CB>
 void conv3_mfma_cb(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
                    int cout, int H, const _Float16* mask) {
-  const unsigned nwg = grid.x * grid.y * grid.z; /* (tiles, output-channel blocks, images) */
-  const bool pf = nwg < 4096;
-  if (pf) k_conv3_mfma<RELU, CB, true, true><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
-  else k_conv3_mfma<RELU, CB, false, true><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  /* grid = (tiles, 16-channel output blocks, images); a workgroup takes
+   * OCB of the blocks (each wave all of them on its 4 pixel rows: the input
+   * operand read from LDS serves OCB products) while that leaves at least
+   * VW_CONV_MIN_WG workgroups */
+  const unsigned nb16 = grid.y, rest = grid.x * grid.z;
+  int ocb = 1;
+  if (nb16 % 4 == 0 && rest * (nb16 / 4) >= VW_CONV_MIN_WG) ocb = 4;
+  else if (nb16 % 2 == 0 && rest * (nb16 / 2) >= VW_CONV_MIN_WG) ocb = 2;
+  const unsigned nwg = rest * (nb16 / ocb);
+  const bool pf = ocb == 1 && nwg < 4096;
+  if (pf) k_conv3_mfma<RELU, CB, true, true, 1><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else if (ocb == 1) k_conv3_mfma<RELU, CB, false, true, 1><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else if (ocb == 2) k_conv3_mfma<RELU, CB, false, true, 2><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
+  else k_conv3_mfma<RELU, CB, false, true, 4><<<nwg, 256, 0, c->st>>>(in, Wk, B, out, cin, cout, H, mask);
 }
 template <bool RELU>
 void conv3_mfma(Net* c, dim3 grid, const _Float16* in, const _Float16* Wk, const float* B, _Float16* out, int cin,
