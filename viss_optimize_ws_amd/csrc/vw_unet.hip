@@ -129,7 +129,12 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
                                                     int cout, int H, const _Float16* __restrict__ mask) {
   static_assert(CB == 16 || CB == 32, "channels per LDS chunk");
   constexpr int STEPS = CB == 32 ? 9 : 5;
-  __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * 40]; /* [y][x][ci], rows of 32 (+8 pad) */
+  /* [y][x][ci], pixel rows of CB channels + 8 halves of padding (the 16
+   * pixels a 16-lane read group takes then fall on disjoint banks); 16-channel
+   * chunks need only 15.5 KB, which lets 8 workgroups share a CU */
+  constexpr int LR = CB + 8;
+  __shared__ __attribute__((aligned(16))) _Float16 tile[18 * 18 * LR];
+  static_assert(18 * 18 * LR >= 16 * 16 * 24, "output staging area");
   const int tpr = (H + 15) / 16;
   /* one-dimensional grid, workgroups ordered (output-channel block, tile x,
    * tile y, image) from fastest and mapped so that consecutive ones run on
@@ -186,9 +191,9 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
           v0 = *(const vw_h8*)s0;
           v1 = *(const vw_h8*)(s0 + HH);
         }
-        _Float16* d = tile + (yy * 18 + 1 + 8 * hx) * 40 + 2 * cp;
+        _Float16* d = tile + (yy * 18 + 1 + 8 * hx) * LR + 2 * cp;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) *(vw_h2*)(d + i * 40) = vw_h2{v0[i], v1[i]};
+        for (int i = 0; i < 8; ++i) *(vw_h2*)(d + i * LR) = vw_h2{v0[i], v1[i]};
       }
       /* halo columns tx0 - 1 and tx0 + 16 */
       for (int idx = threadIdx.x; idx < cb * 36; idx += 256) {
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
         const int gy = ty0 + yy - 1, gx = side ? tx0 + 16 : tx0 - 1;
         _Float16 v = (_Float16)0.f;
         if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
-        tile[(yy * 18 + (side ? 17 : 0)) * 40 + c] = v;
+        tile[(yy * 18 + (side ? 17 : 0)) * LR + c] = v;
       }
     } else {
       for (int idx = threadIdx.x; idx < 324 * cb; idx += 256) {
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
         const int gy = ty0 + yy - 1, gx = tx0 + xx - 1;
         _Float16 v = (_Float16)0.f;
         if (gy >= 0 && gy < H && gx >= 0 && gx < H) v = in[((int64_t)n * cin + c0 + c) * HH + (int64_t)gy * H + gx];
-        tile[(yy * 18 + xx) * 40 + c] = v;
+        tile[(yy * 18 + xx) * LR + c] = v;
       }
     }
     __syncthreads();
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(const _Float16* __restrict__
     const int ky = kon ? tap / 3 : 0, kx = kon ? tap % 3 : 0;                                   \
     _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                             \
       const int py = 4 * wave + j; /* tile row of this column block */                          \
-      vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * 40 + ci0);                  \
+      vw_h8 bv = *(const vw_h8*)(tile + ((py + ky) * 18 + r + kx) * LR + ci0);                  \
       if (!kon) bv = vw_h8{};                                                                   \
       _Pragma("unroll") for (int b = 0; b < OCB; ++b)                                            \
         acc[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av_[b], bv, acc[b][j], 0, 0, 0);      \
