@@ -398,6 +398,25 @@ def run(pairs=20000, heldout=512, nside=64, img_n=256, batch=16, epochs=2.0, lr=
     }
 
 
+def time_training(img_n=256, batch=16, steps=200, fp16=True, seed=1):
+    """training throughput alone: a resident set of 16 batches of random
+    images, 20 untimed steps, then `steps` timed Adam steps (wall clock around
+    vw_unet_train, which returns after its stream is drained)"""
+    rng = np.random.default_rng(seed)
+    n = 16 * batch
+    X = rng.standard_normal((n, img_n, img_n)).astype(np.float32)
+    Y = rng.standard_normal((n, img_n, img_n)).astype(np.float32)
+    with UNet(img_n, batch, fp16=fp16, seed=seed) as net:
+        net.set_data(X, Y)
+        net.train(20, batch, 1e-3, seed=0)
+        t = time.perf_counter()
+        losses = net.train(steps, batch, 1e-3, seed=1)
+        dt = time.perf_counter() - t
+    return {"img_n": img_n, "batch": batch, "steps": steps, "activations": "fp16" if fp16 else "fp32",
+            "ms_per_step": round(1e3 * dt / steps, 4), "images_per_s": round(steps * batch / dt, 1),
+            "loss_last": float(losses[-1])}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--pairs", type=int, default=20000)
@@ -411,7 +430,12 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--gate-samples", type=int, default=64)
     ap.add_argument("--refine-iters", type=int, default=50, help="CGLS iterations of the data-consistency refinement")
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="only time this many training steps on random images (no pairs, no gate) and exit")
     a = ap.parse_args(argv)
+    if a.time_steps > 0:
+        print(json.dumps(time_training(a.img_n, a.batch, a.time_steps, not a.fp32, a.seed)), flush=True)
+        return
     out = run(a.pairs, a.heldout, a.nside, a.img_n, a.batch, a.epochs, a.lr, not a.fp32, a.seed,
               gate_samples=a.gate_samples, refine_iters=a.refine_iters,
               log=lambda m: print(m, file=sys.stderr, flush=True))
