@@ -316,23 +316,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISSB_WA
       const double* pz_ = px[buf][2];
       const double* pn_ = px[buf][3];
       float cv[8], sv[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int kk = g * 8 + i;
-        const double pl = pl_[kk], pm = pm_[kk], pz = pz_[kk];
-        const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl, pm, pz);
-        const float f = (float)__builtin_amdgcn_fract(x);
-        const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
-        bool on = cls == VW_CLS_VISIBLE;
-        if (any_mixed && cls == VW_CLS_MIXED) {
-          const double pn = pn_[kk];
-          const int w_ = wave * 16 + r;
-          on = vw_visible(pl, pm, pn, frs[0][w_], frs[1][w_], frs[2][w_], frs[3][w_], frs[4][w_]) &&
-               vw_visible(pl, pm, pn, frs[5][w_], frs[6][w_], frs[7][w_], frs[8][w_], frs[9][w_]);
-        }
-        cv[i] = on ? cs : 0.0f;
-        sv[i] = on ? sn : 0.0f;
+      /* two forms of the step, chosen per wave: without mixed rows a lane's
+       * mask is its row's class alone (no per-term branch); with them every
+       * lane evaluates both antennas' predicates, branch-free */
+#define VW_PHASOR(MASKED_)                                                                              \
+  do {                                                                                                  \
+    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                     \
+      const int kk = g * 8 + i;                                                                         \
+      const double pl = pl_[kk], pm = pm_[kk], pz = pz_[kk];                                            \
+      const double x = vw_viss_phase(rec.u, rec.v, rec.w, pl, pm, pz);                                  \
+      const float f = (float)__builtin_amdgcn_fract(x);                                                 \
+      const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);                         \
+      bool on = cls == VW_CLS_VISIBLE;                                                                  \
+      if (MASKED_) {                                                                                    \
+        const double pn = pn_[kk];                                                                      \
+        const int va = vw_visible(pl, pm, pn, fa[0], fa[1], fa[2], fa[3], fa[4]);                       \
+        const int vb = vw_visible(pl, pm, pn, fb[0], fb[1], fb[2], fb[3], fb[4]);                       \
+        on = on | ((cls == VW_CLS_MIXED) & ((va & vb) != 0));                                           \
+      }                                                                                                 \
+      cv[i] = on ? cs : 0.0f;                                                                           \
+      sv[i] = on ? sn : 0.0f;                                                                           \
+    }                                                                                                   \
+  } while (0)
+      if (any_mixed) {
+        const int w_ = wave * 16 + r;
+        const double fa[5] = {frs[0][w_], frs[1][w_], frs[2][w_], frs[3][w_], frs[4][w_]};
+        const double fb[5] = {frs[5][w_], frs[6][w_], frs[7][w_], frs[8][w_], frs[9][w_]};
+        VW_PHASOR(true);
+      } else {
+        const double fa[5] = {}, fb[5] = {};
+        VW_PHASOR(false);
       }
+#undef VW_PHASOR
       vw_bf16x8 ach, acl, ash, asl;
       vw_split8(cv, ach, acl);
       vw_split8(sv, ash, asl);
