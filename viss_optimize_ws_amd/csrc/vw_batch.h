@@ -610,22 +610,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
     }
     /* positions of the lane: 8 g .. 8 g + 7 of the k-step */
     const int64_t jk = qa + 32 * k + g * 8;
-    uint64_t cw = *(const uint64_t*)(&gcl[buf][g * 8]);
+    const uint64_t cw = *(const uint64_t*)(&gcl[buf][g * 8]);
     const vw_u32x4 q03 = *(const vw_u32x4*)(&gq[buf][g * 8]), q47 = *(const vw_u32x4*)(&gq[buf][g * 8 + 4]);
-    bool vis8 = false, all8 = jk + 8 <= qe;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int cls = (int)((cw >> (8 * i)) & 0xffu);
-      const int64_t q = (int32_t)(i < 4 ? q03[i & 3] : q47[i & 3]);
-      vis8 |= jk + i < qe && cls != VW_CLS_HIDDEN;
-      all8 = all8 && cls == VW_CLS_VISIBLE && q >= q0;
-    }
     /* a k-step whose groups are all hidden from this tile adds nothing; one
-     * whose groups are all in range, visible and of nonzero key (every lane)
-     * needs no per-term mask */
-    if (wave_on && __ballot(vis8) != 0) {
+     * whose groups are all in range, visible and of nonzero key needs no
+     * per-term mask.  Lane l < 32 tests position l of the step, so that the
+     * two step flags cost one test per lane and two ballots */
+    const int lt = lane & 31;
+    const int cl_ = (int)gcl[buf][lt];
+    const bool in_ = qa + 32 * k + lt < qe;
+    const bool live1 = lane < 32 && in_ && cl_ != VW_CLS_HIDDEN;
+    const bool bad1 = lane < 32 && !(in_ && cl_ == VW_CLS_VISIBLE && (int64_t)gq[buf][lt] >= q0);
+    if (wave_on && __ballot(live1) != 0) {
       float cv[8], sv[8];
-      if (__ballot(!all8) == 0) {
+      if (__ballot(bad1) == 0) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           const double2 u = grec[buf][g * 8 + i];
