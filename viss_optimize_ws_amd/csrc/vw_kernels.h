@@ -470,9 +470,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
         bool vis[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-          vis[k] = MIXED ? (ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                            vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc))
-                         : vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
+          vis[k] = vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
         if (MIXED) {
 #pragma unroll
           for (int k = 0; k < PPL; ++k)
@@ -1276,6 +1274,14 @@ struct ReconAccF32 {
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
     b = fmaf(vrf, cs, fmaf(-vif, sn, b));
   }
+  /* the same, kept only where use holds: a select, not a branch */
+  __device__ void term_if(bool use, double ugu, double vgu, float vrf, float vif, double lp, double mp) {
+    const double x = vw_recon_phase(ugu, vgu, lp, mp);
+    const float f = (float)__builtin_amdgcn_fract(x);
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+    const float nb = fmaf(vrf, cs, fmaf(-vif, sn, b));
+    b = use ? nb : b;
+  }
   __device__ void fold() {
     float y = b - c;
     float t = s + y;
@@ -1302,8 +1308,7 @@ struct ReconLane {
                                       const double* lp, const double* mp, const double2* tab) {
     if (MIXED) {
 #pragma unroll
-      for (int k = 0; k < PPL; ++k)
-        if (use[k]) f[k].term(ugu, vgu, (float)vr[k], (float)vi[k], lp[k], mp[k]);
+      for (int k = 0; k < PPL; ++k) f[k].term_if(use[k], ugu, vgu, (float)vr[k], (float)vi[k], lp[k], mp[k]);
     } else {
       double fr[PPL];
       int ix[PPL];
@@ -1430,9 +1435,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
             bool vis[PPL];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-              vis[k] = MIXED ? (ok[k] && vw_visible(pl[k], pm[k], pn[k], ax, ay, az, ai, ac) &&
-                                vw_visible(pl[k], pm[k], pn[k], bx, by, bz, bi, bc))
-                             : vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
+              vis[k] = vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
               wl[k] += vis[k] ? 1 : 0;
             }
             acc.add(g.ugu, g.vgu, vr, vi, vis, lp, mp, tab);
