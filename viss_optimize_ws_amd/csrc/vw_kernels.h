@@ -377,7 +377,7 @@ __device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const dou
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     double sy, cy, sn, cs;
-    vw_tab_poly(f[k], &sy, &cy);
+    vw_tab_poly_short(f[k], &sy, &cy);
     vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
     dre = fma(bw[k], cs, dre);
     dim = fma(bw[k], sn, dim);
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256) void k_viss_adj(VissAdjArgs a) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
         double sy, cy, sn, cs;
-        vw_tab_poly(f[k], &sy, &cy);
+        vw_tab_poly_short(f[k], &sy, &cy);
         vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
         /* Re(r exp(+j x)) = r.x cos x - r.y sin x */
         const double t = fma(r.x, cs, fma(-r.y, sn, acc[k]));
@@ -770,7 +770,7 @@ __device__ __forceinline__ uint64_t vf_tile(const VfTileLds<TP>& px, const VfLan
         const double f = vw_tab_split(vw_viss_phase(L.kx[gi], L.ky[gi], L.kz[gi], l, m, nm1), &ix);
         const double2 te = tab[ix];
         double sy, cy, sd, cd;
-        vw_tab_poly(f, &sy, &cy);
+        vw_tab_poly_short(f, &sy, &cy);
         vw_tab_rotate(te.x, te.y, sy, cy, &sd, &cd);
         c = (T)cd;
         s = (T)sd;
@@ -1320,7 +1320,7 @@ struct ReconLane {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
         double sy, cy, sn, cs;
-        vw_tab_poly(fr[k], &sy, &cy);
+        vw_tab_poly_short(fr[k], &sy, &cy);
         vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
         const double t = vw_recon_add(d[k], vr[k], vi[k], sn, cs);
         d[k] = use[k] ? t : d[k];

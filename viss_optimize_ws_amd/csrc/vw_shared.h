@@ -160,6 +160,21 @@ VW_HD void vw_tab_poly(double f, double* sy, double* cy) {
   *cy = fma(z, pc, 1.0);
 }
 
+/* The kernels' form of the same polynomials, one degree lower in each:
+ *   sin = f (a - f^2 a^3/6),  cos = 1 - f^2 a^2/2   (y = a f, |y| <= pi/N)
+ * at N = 1024 the omitted terms are below y^5/120 <= 2.3e-15 (sin) and
+ * y^4/24 <= 3.8e-12 (cos) in absolute value per term, far inside the plain
+ * form's 1e-9 bound on the outputs, for two FP64 FMAs less per term (the
+ * oracle keeps the full polynomials). */
+#if VW_TAB_LOG2 != 10
+#error "the kernels' short polynomials are costed for a 1024-entry table"
+#endif
+VW_HD void vw_tab_poly_short(double f, double* sy, double* cy) {
+  const double z = f * f;
+  *sy = f * fma(z, -0x1.4abbce625be53p-19 / VW_TAB_SH(3), 0x1.921fb54442d18p-6 / VW_TAB_SH(1));
+  *cy = fma(z, -0x1.3bd3cc9be45dep-12 / VW_TAB_SH(2), 1.0);
+}
+
 /* rotation by the table entry (ec, es) */
 VW_HD void vw_tab_rotate(double ec, double es, double sy, double cy, double* s_out, double* c_out) {
   *s_out = fma(es, cy, ec * sy);
