@@ -70,11 +70,11 @@ FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # FP64 lane-instructions per second (1
 # FP64 flops of one evaluated term in the plain forms as executed (fma = 2,
 # table index not counted).  Viss: phase u*l + v*m + w*(n-1) on operands
 # pre-scaled by 1024, 1 mul + 2 fma = 5; split t + M - M, t - k = 3; z = f^2
-# 1; sin f*(a + z*(b + z*c)) 2 fma + mul = 5; cos 2 fma = 4; rotation by the
+# 1; sin f*(a + z*b) fma + mul = 3; cos 1 + z*c, 1 fma = 2; rotation by the
 # table entry 2 mul + 2 fma = 6; accumulation 2 fma = 4.  Recon: phase 1 mul
-# + 1 fma = 3, then as Viss from the split on.  Instructions: 18 and 17.
-VISS_TERM_FLOPS_PLAIN, VISS_TERM_INSTR_PLAIN = 28, 18
-RECON_TERM_FLOPS_PLAIN, RECON_TERM_INSTR_PLAIN = 26, 17
+# + 1 fma = 3, then as Viss from the split on.  Instructions: 16 and 15.
+VISS_TERM_FLOPS_PLAIN, VISS_TERM_INSTR_PLAIN = 24, 16
+RECON_TERM_FLOPS_PLAIN, RECON_TERM_INSTR_PLAIN = 22, 15
 # instructions of one evaluated term of the mixed recon as written
 # (ReconAccF32::term): phase mul_f64 + fma_f64, fract_f64, cvt_f32_f64,
 # sin_f32, cos_f32, 2 fma_f32; weighted by their measured issue costs
