@@ -70,10 +70,13 @@ FP64_INSTR_PEAK_T = FP64_PEAK_TFLOPS / 2  # FP64 lane-instructions per second (1
 # FP64 flops of one evaluated term in the plain forms as executed (fma = 2,
 # table index not counted).  Viss: phase u*l + v*m + w*(n-1) on operands
 # pre-scaled by 1024, 1 mul + 2 fma = 5; split t + M - M, t - k = 3; z = f^2
-# 1; sin f*(a + z*b) fma + mul = 3; cos 1 + z*c, 1 fma = 2; rotation by the
-# table entry 2 mul + 2 fma = 6; accumulation 2 fma = 4.  Recon: phase 1 mul
-# + 1 fma = 3, then as Viss from the split on.  Instructions: 16 and 15.
-VISS_TERM_FLOPS_PLAIN, VISS_TERM_INSTR_PLAIN = 24, 16
+# 1; the pixel weight folded into the coefficients: w cos = w0 + z*w2, 1 fma
+# = 2, w sin = f*(w1 + z*w3), fma + mul = 3; rotation by the table entry
+# accumulated directly, 4 fma = 8: 22 flops, 14 instructions.  Recon: phase
+# 1 mul + 1 fma = 3; split 3; z 1; sin fma + mul = 3; cos 1 fma = 2;
+# rotation 2 mul + 2 fma = 6; accumulation 2 fma = 4: 22 flops, 15
+# instructions.
+VISS_TERM_FLOPS_PLAIN, VISS_TERM_INSTR_PLAIN = 22, 14
 RECON_TERM_FLOPS_PLAIN, RECON_TERM_INSTR_PLAIN = 22, 15
 # instructions of one evaluated term of the mixed recon as written
 # (ReconAccF32::term): phase mul_f64 + fma_f64, fract_f64, cvt_f32_f64,
@@ -256,7 +259,7 @@ def one_sky_leg(workload, steps, warmup, serial, rates, mixed=True, plain_ant=Fa
             fp64_instr_issue_frac=round(RECON_TERM_INSTR_PLAIN * ev_recon / (recon_avg * 1e-3) / 1e12 / FP64_INSTR_PEAK_T, 4),
             launches_per_step=nr),
         "roofline_viss": _roof(
-            "k_viss<256,16,false,false> (plain per-row Viss)", "valu_fp64", "FP64 vector ALU (published FP64 peak)",
+            "k_viss_plain<256,8,false> (plain per-row Viss)", "valu_fp64", "FP64 vector ALU (published FP64 peak)",
             VISS_TERM_FLOPS_PLAIN * ev_viss, viss_avg, FP64_PEAK_TFLOPS, "TFLOP/s",
             f"{VISS_TERM_FLOPS_PLAIN} FP64 flops x {ev_viss:.6g} evaluated (half row, pixel) terms per launch",
             _traffic(workload, "k_viss_plain"),

@@ -390,9 +390,10 @@ struct VissShape {
   int64_t tpc, nchunk, nrb, bpc, nblocks;
 };
 /* half rows per wave: as many accumulators as fit without spilling
- * (the counting variants carry one more register per row) */
+ * (the counting variants carry one more register per row); the plain form
+ * also holds four weight coefficients per pixel */
 #ifndef VW_VISS_RPW_PLAIN
-#define VW_VISS_RPW_PLAIN 16
+#define VW_VISS_RPW_PLAIN 8
 #endif
 #ifndef VW_VISS_RPW_MIXED
 #define VW_VISS_RPW_MIXED 16
@@ -828,7 +829,10 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
 
 template <int TP, int RPW, bool MIXED, bool COUNT>
 void launch_viss(const VissArgs& a, unsigned nb, hipStream_t st) {
-  k_viss<TP, RPW, MIXED, COUNT><<<nb, 256, 0, st>>>(a);
+  if constexpr (MIXED)
+    k_viss<TP, RPW, COUNT><<<nb, 256, 0, st>>>(a);
+  else
+    k_viss_plain<TP, RPW, COUNT><<<nb, 256, 0, st>>>(a);
 }
 
 /* one tile pass of the antenna-factorized Viss: cone bits per (tile, frame),

@@ -363,10 +363,18 @@ __device__ __forceinline__ void vw_viss_term_mixed(const RowRec& rec, double l, 
 /* plain form, the PPL terms of one lane for one half row: phases (operands
  * pre-scaled by 1024), all table indices, the table reads, then polynomials
  * and rotations, so the LDS reads overlap the arithmetic */
-template <int PPL>
+/* The pixel weight bw is folded into the polynomial coefficients once per
+ * sub-tile: w0 = bw, w1 = bw a, w2 = bw (-a^2/2), w3 = bw (-a^3/6) give
+ *   bw cy = w0 + w2 z,  bw sy = f (w1 + w3 z),
+ * and the rotation by the table entry (ec, es) is accumulated directly:
+ *   dre += ec (bw cy) - es (bw sy),  dim += es (bw cy) + ec (bw sy),
+ * four FMAs, two products fewer per term than rotating, then weighting.
+ * MASK: a pixel with vis[k] false adds nothing (selects on bw cy, bw sy). */
+template <int PPL, bool MASK>
 __device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const double* pl, const double* pm,
-                                                    const double* pz, const double* bw, const double2* tab,
-                                                    double& dre, double& dim) {
+                                                    const double* pz, const double* w0, const double* w1,
+                                                    const double* w2, const double* w3, const bool* vis,
+                                                    const double2* tab, double& dre, double& dim) {
   double f[PPL];
   int ix[PPL];
 #pragma unroll
@@ -376,35 +384,35 @@ __device__ __forceinline__ void vw_viss_terms_plain(const double4& rt, const dou
   for (int k = 0; k < PPL; ++k) e[k] = tab[ix[k]];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    double sy, cy, sn, cs;
-    vw_tab_poly_short(f[k], &sy, &cy);
-    vw_tab_rotate(e[k].x, e[k].y, sy, cy, &sn, &cs);
-    dre = fma(bw[k], cs, dre);
-    dim = fma(bw[k], sn, dim);
+    const double z = f[k] * f[k];
+    double cw = fma(z, w2[k], w0[k]);
+    double sw = f[k] * fma(z, w3[k], w1[k]);
+    if (MASK) {
+      cw = vis[k] ? cw : 0.0;
+      sw = vis[k] ? sw : 0.0;
+    }
+    dre = fma(-e[k].y, sw, fma(e[k].x, cw, dre));
+    dim = fma(e[k].x, sw, fma(e[k].y, cw, dim));
   }
 }
 
-template <int TP, int RPW, bool MIXED, bool COUNT>
-/* occupancy: the plain form keeps 16 FP64 row sums per lane at 3 waves per
- * SIMD; the mixed form (16 FP32 row sums) runs 5 (measured best) */
+/* occupancy: the plain form keeps 8 FP64 row sums and 4 weight
+ * coefficients per pixel per lane at 3 waves per SIMD; the mixed form (16
+ * FP32 row sums) runs 5 (measured best) */
 #ifndef VW_VISS_WAVES_PLAIN
 #define VW_VISS_WAVES_PLAIN 3
 #endif
 #ifndef VW_VISS_WAVES_MIXED
 #define VW_VISS_WAVES_MIXED 5
 #endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_VISS_WAVES_MIXED
-                                                                            : VW_VISS_WAVES_PLAIN))) void k_viss(VissArgs a) {
+/* per-row Viss, mixed form (the plain form is k_viss_plain) */
+template <int TP, int RPW, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISS_WAVES_MIXED))) void k_viss(VissArgs a) {
   /* a tile is walked in sub-tiles of 256 pixels (4 per lane), so the
    * register footprint does not grow with the tile size */
   constexpr int PPL = 4;
   constexpr int NSUB = TP / 256;
   static_assert(RPW % 4 == 0, "class words hold 4 rows");
-  __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
-  if (!MIXED) {
-    vw_tab_fill(tab);
-    __syncthreads();
-  }
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t chunk = vb / a.bpc;
   const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
@@ -416,10 +424,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
   const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
 
   float fre[RPW], fim[RPW];
-  double dre[RPW], dim[RPW];
   int cnt[RPW];
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) { fre[r] = 0.f; fim[r] = 0.f; dre[r] = 0.0; dim[r] = 0.0; cnt[r] = 0; }
+  for (int r = 0; r < RPW; ++r) { fre[r] = 0.f; fim[r] = 0.f; cnt[r] = 0; }
 
   for (int64_t st = t0 * NSUB; st < t1 * NSUB; ++st) {
     const int64_t tile = st / NSUB;
@@ -428,7 +435,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
     const uint32_t* cwp = a.cls + (((tile - a.tile0) * a.nh_pad + r0) >> 2);
 #pragma unroll
     for (int j = 0; j < RPW / 4; ++j) cw[j] = (uint32_t)vw_rfl((int)cwp[j]);
-    double pl[PPL], pm[PPL], pn[PPL], pz[PPL], pb[PPL];
+    double pl[PPL], pm[PPL], pn[PPL], pz[PPL];
     float pbf[PPL];
     bool ok[PPL];
 #pragma unroll
@@ -441,8 +448,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
       pn[k] = a.n[pp];
       pz[k] = a.nm1[pp];
       /* weights are zero outside the sky, so the visible path needs no mask */
-      if (MIXED) pbf[k] = ok[k] ? a.Bf[pp] : 0.0f;
-      else pb[k] = ok[k] ? a.B[pp] : 0.0;
+      pbf[k] = ok[k] ? a.Bf[pp] : 0.0f;
     }
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
@@ -450,13 +456,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
       /* rows past nh carry the hidden class */
       const int cls = (int)((cw[r >> 2] >> (8 * (r & 3))) & 0xffu);
       if (cls == VW_CLS_VISIBLE) {
-        if (MIXED) {
-          const RowRec rec = a.rows[h];
+        const RowRec rec = a.rows[h];
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], pbf[k], fre[r], fim[r]);
-        } else {
-          vw_viss_terms_plain<PPL>(a.rowt[h], pl, pm, pz, pb, tab, dre[r], dim[r]);
-        }
+        for (int k = 0; k < PPL; ++k) vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], pbf[k], fre[r], fim[r]);
         if (COUNT) {
 #pragma unroll
           for (int k = 0; k < PPL; ++k) cnt[r] += ok[k] ? 1 : 0;
@@ -471,16 +473,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
           vis[k] = vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
-        if (MIXED) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k)
-            vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], vis[k] ? pbf[k] : 0.f, fre[r], fim[r]);
-        } else {
-          double bw[PPL];
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) bw[k] = vis[k] ? pb[k] : 0.0;
-          vw_viss_terms_plain<PPL>(a.rowt[h], pl, pm, pz, bw, tab, dre[r], dim[r]);
-        }
+        for (int k = 0; k < PPL; ++k)
+          vw_viss_term_mixed(rec, pl[k], pm[k], pz[k], vis[k] ? pbf[k] : 0.f, fre[r], fim[r]);
         if (COUNT) {
 #pragma unroll
           for (int k = 0; k < PPL; ++k) cnt[r] += vis[k] ? 1 : 0;
@@ -492,10 +487,159 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? VW_
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int64_t h = r0 + r;
-    double re = MIXED ? (double)fre[r] : dre[r];
-    double im = MIXED ? (double)fim[r] : dim[r];
-    re = vw_wave_sum(re);
-    im = vw_wave_sum(im);
+    double re = vw_wave_sum((double)fre[r]);
+    double im = vw_wave_sum((double)fim[r]);
+    int64_t cc = 0;
+    if (COUNT) cc = vw_wave_sum_i((int64_t)cnt[r]);
+    if (lane == 0 && h < a.nh) {
+      a.part[chunk * a.nh_pad + h] = make_double2(re, -im);
+      if (COUNT) a.cpart[chunk * a.nh_pad + h] = cc;
+    }
+  }
+}
+
+/* Plain per-row Viss with the pixel data prefetched into LDS.  Each wave
+ * keeps its own LDS copy of the next sub-tile's l, m, n and weights, written
+ * by direct global-to-LDS loads (16 bytes a lane: two neighbouring pixels)
+ * while the current sub-tile is summed, together with the next class words;
+ * a sub-tile then starts with LDS reads, and no registers hold the data in
+ * flight.  Nothing else in the sub-tile loop reads global memory, so the
+ * loop never waits on the prefetch: the wave's rows are read once up front
+ * ((u, v, w) into scalar registers, the antenna data of the occlusion
+ * predicates into LDS).  Lane i owns pixels 128 j + 2 i + e (j, e in {0, 1})
+ * of a 256-pixel sub-tile; pixel counts are even, so a pair is in the sky or
+ * out of it whole, and pairs outside it read pixels 0 and 1 under a zero
+ * weight.  n - 1 is formed as in k_pix2lmn.  Terms, masks and partials as
+ * k_viss. */
+template <int TP, int RPW, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISS_WAVES_PLAIN))) void k_viss_plain(
+    VissArgs a) {
+  constexpr int PPL = 4;
+  constexpr int SUB = 64 * PPL;
+  constexpr int NSUB = TP / SUB;
+  constexpr int NCW = RPW / 4; /* class words per sub-tile */
+  static_assert(RPW % 4 == 0 && NCW <= 64, "class words hold 4 rows");
+  static_assert(TP % SUB == 0, "tiles hold whole sub-tiles");
+  __shared__ double2 tab[VW_TAB_N];
+  __shared__ double2 pre[4][4][2][64]; /* [wave][l, m, n, B][half][lane] */
+  __shared__ double rant[4][RPW][10];  /* [wave][row][antenna a (x y z i c), b (...)] */
+  vw_tab_fill(tab);
+  __syncthreads();
+  const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t chunk = vb / a.bpc;
+  const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int64_t rb = (vb - chunk * a.bpc) * 4 + wave;
+  if (chunk >= a.nchunk || rb >= a.nrb) return;
+  const int64_t r0 = rb * RPW; /* r0 + RPW <= nh_pad: nh_pad is a multiple of 64 */
+  const int64_t t0 = a.tile0 + chunk * a.tpc;
+  const int64_t tend = a.tile0 + a.ntile;
+  const int64_t t1 = (t0 + a.tpc < tend) ? t0 + a.tpc : tend;
+  const int64_t s0 = t0 * NSUB, s1 = t1 * NSUB;
+
+  /* the wave's rows; rows past nh are hidden and never summed */
+  double4 rt[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) rt[r] = (r0 + r < a.nh) ? a.rowt[r0 + r] : make_double4(0.0, 0.0, 0.0, 0.0);
+  if (lane < RPW && r0 + lane < a.nh) {
+    const RowRec rec = a.rows[r0 + lane];
+    const double* qa = a.ant8 + (int64_t)rec.ia * 8;
+    const double* qb = a.ant8 + (int64_t)rec.ib * 8;
+    double* d = rant[wave][lane];
+    d[0] = qa[0]; d[1] = qa[1]; d[2] = qa[2]; d[3] = qa[3]; d[4] = qa[7];
+    d[5] = qb[0]; d[6] = qb[1]; d[7] = qb[2]; d[8] = qb[3]; d[9] = qb[7];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  typedef __attribute__((address_space(3))) void* lds_ptr;
+  uint32_t cwn = 0; /* lane j < NCW: class word j of the prefetched sub-tile */
+  auto prefetch = [&](int64_t s) {
+    const double* src[4] = {a.l, a.m, a.n, a.B};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t p = s * SUB + 128 * j + 2 * lane;
+      const int64_t pp = p < a.npix ? p : 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        __builtin_amdgcn_global_load_lds((const void*)(src[q] + pp), (lds_ptr)&pre[wave][q][j][0], 16, 0, 0);
+    }
+    const uint32_t* cwp = a.cls + ((((s / NSUB) - a.tile0) * a.nh_pad + r0) >> 2);
+    cwn = cwp[lane < NCW ? lane : 0];
+  };
+
+  double dre[RPW], dim[RPW];
+  int cnt[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) { dre[r] = 0.0; dim[r] = 0.0; cnt[r] = 0; }
+
+  prefetch(s0);
+  for (int64_t st = s0; st < s1; ++st) {
+    /* this sub-tile's prefetch has landed */
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    uint32_t cw[NCW];
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) cw[j] = (uint32_t)__builtin_amdgcn_readlane((int)cwn, j);
+    double pl[PPL], pm[PPL], pn[PPL], pz[PPL], w0[PPL], w1[PPL], w2[PPL], w3[PPL];
+    bool ok[PPL];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double2 L = pre[wave][0][j][lane], M = pre[wave][1][j][lane];
+      const double2 N = pre[wave][2][j][lane], W = pre[wave][3][j][lane];
+      pl[2 * j] = L.x;
+      pl[2 * j + 1] = L.y;
+      pm[2 * j] = M.x;
+      pm[2 * j + 1] = M.y;
+      pn[2 * j] = N.x;
+      pn[2 * j + 1] = N.y;
+      const bool okj = st * SUB + 128 * j + 2 * lane < a.npix;
+      ok[2 * j] = okj;
+      ok[2 * j + 1] = okj;
+      w0[2 * j] = okj ? W.x : 0.0;
+      w0[2 * j + 1] = okj ? W.y : 0.0;
+    }
+    /* the reads are done before the buffer is refilled */
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if (st + 1 < s1) prefetch(st + 1);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      pz[k] = pn[k] - 1.0;
+      w1[k] = w0[k] * VW_SHORT_S1;
+      w2[k] = w0[k] * VW_SHORT_C2;
+      w3[k] = w0[k] * VW_SHORT_S3;
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      /* rows past nh carry the hidden class */
+      const int cls = (int)((cw[r >> 2] >> (8 * (r & 3))) & 0xffu);
+      if (cls == VW_CLS_VISIBLE) {
+        vw_viss_terms_plain<PPL, false>(rt[r], pl, pm, pz, w0, w1, w2, w3, ok, tab, dre[r], dim[r]);
+        if (COUNT) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) cnt[r] += ok[k] ? 1 : 0;
+        }
+      } else if (cls == VW_CLS_MIXED) {
+        const double* d = rant[wave][r];
+        const double ax = d[0], ay = d[1], az = d[2], ai = d[3], ac = d[4];
+        const double bx = d[5], by = d[6], bz = d[7], bi = d[8], bc = d[9];
+        bool vis[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+          vis[k] = vw_vis2<true>(ok[k], pl[k], pm[k], pn[k], ax, ay, az, ai, ac, bx, by, bz, bi, bc);
+        vw_viss_terms_plain<PPL, true>(rt[r], pl, pm, pz, w0, w1, w2, w3, vis, tab, dre[r], dim[r]);
+        if (COUNT) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) cnt[r] += vis[k] ? 1 : 0;
+        }
+      }
+    }
+  }
+  /* one cross-lane tree per row */
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int64_t h = r0 + r;
+    const double re = vw_wave_sum(dre[r]);
+    const double im = vw_wave_sum(dim[r]);
     int64_t cc = 0;
     if (COUNT) cc = vw_wave_sum_i((int64_t)cnt[r]);
     if (lane == 0 && h < a.nh) {

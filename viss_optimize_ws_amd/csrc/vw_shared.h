@@ -169,10 +169,13 @@ VW_HD void vw_tab_poly(double f, double* sy, double* cy) {
 #if VW_TAB_LOG2 != 10
 #error "the kernels' short polynomials are costed for a 1024-entry table"
 #endif
+#define VW_SHORT_S1 (0x1.921fb54442d18p-6 / VW_TAB_SH(1))   /* a */
+#define VW_SHORT_S3 (-0x1.4abbce625be53p-19 / VW_TAB_SH(3)) /* -a^3/6 */
+#define VW_SHORT_C2 (-0x1.3bd3cc9be45dep-12 / VW_TAB_SH(2)) /* -a^2/2 */
 VW_HD void vw_tab_poly_short(double f, double* sy, double* cy) {
   const double z = f * f;
-  *sy = f * fma(z, -0x1.4abbce625be53p-19 / VW_TAB_SH(3), 0x1.921fb54442d18p-6 / VW_TAB_SH(1));
-  *cy = fma(z, -0x1.3bd3cc9be45dep-12 / VW_TAB_SH(2), 1.0);
+  *sy = f * fma(z, VW_SHORT_S3, VW_SHORT_S1);
+  *cy = fma(z, VW_SHORT_C2, 1.0);
 }
 
 /* rotation by the table entry (ec, es) */
