@@ -448,7 +448,10 @@ VissFShape viss_fshape(const Ctx* c, int64_t tlen, int64_t nt, int64_t lim = 0) 
 }
 
 /* recon launch shape: PPL points per lane, group ranges to fill the GPU */
-constexpr int RECON_PPL = 4;
+#ifndef VW_RECON_PPL
+#define VW_RECON_PPL 4
+#endif
+constexpr int RECON_PPL = VW_RECON_PPL;
 struct ReconShape {
   int64_t pwaves, nchunk;
 };

@@ -1503,14 +1503,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   const int64_t pbase = pw * 64 * PPL; /* pass-relative */
   const int64_t tile = pbase / a.TP;   /* pass-relative tile (class tables are per pass) */
   const vw_segparams sp = a.info->sp;
-  const int64_t nu = a.info->nuniq;
-  const int64_t q0 = a.info->first_nz;
+  /* group counts fit 32 bits; uniform values keep the range tests scalar */
+  const int64_t nu = vw_rfl((int)a.info->nuniq);
+  const int64_t q0 = vw_rfl((int)a.info->first_nz);
   /* group range of this chunk, aligned to 16 groups */
   const int64_t base = q0 & ~(int64_t)15;
   const int64_t span = ((nu - base) + a.nchunk - 1) / a.nchunk;
   const int64_t spanA = (span + 15) & ~(int64_t)15;
-  const int64_t qa_ = base + chunk * spanA;
-  const int64_t qb_ = qa_ + spanA < nu ? qa_ + spanA : nu;
+  const int64_t qa_ = vw_rfl((int)(base + chunk * spanA));
+  const int64_t qb_ = vw_rfl((int)(qa_ + spanA < nu ? qa_ + spanA : nu));
 
   double pl[PPL], pm[PPL], pn[PPL], lp[PPL], mp[PPL];
   bool ok[PPL];
@@ -1560,7 +1561,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         for (int j = 0; j < 4; ++j) {
           const int64_t q = q4 + j;
           const int cls = (int)((cw >> (8 * j)) & 0xffu);
-          if (q < q0 || q >= qb_ || cls == VW_CLS_HIDDEN) continue;
+          if ((int)q < (int)q0 || (int)q >= (int)qb_ || cls == VW_CLS_HIDDEN) continue;
           const GRec g = blk[q - q16];
           if (KIND == VW_KIND_REP && a.rep_finite && !(isfinite(g.vr) && isfinite(g.vi))) continue;
           double vr[PPL], vi[PPL];
