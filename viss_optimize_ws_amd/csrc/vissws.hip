@@ -75,6 +75,7 @@ struct Ctx {
   /* segment buffers of the current set (pointers into seg[cur]) */
   RowRec* d_rows = nullptr;
   double4* d_rowt = nullptr; /* half-row (u, v, w) * 1024 for the plain Viss phase */
+  double2* d_tab = nullptr;  /* the plain kernels' sine table (VW_TAB_N entries) */
   double* d_ant8 = nullptr;
   struct SegSet {
     RowRec* rows;
@@ -660,6 +661,8 @@ int create_device(Ctx* c) {
       (rc = dalloc(c, &c->d_B, np)) || (rc = dalloc(c, &c->d_Bf, np)) ||
       (rc = dalloc(c, &c->d_Cacc, np * (size_t)c->nsky)) || (rc = dalloc(c, &c->d_Wacc, np)))
     return rc;
+  if ((rc = dalloc(c, &c->d_tab, (size_t)VW_TAB_N))) return rc;
+  k_tab_build<<<(VW_TAB_N + 255) / 256, 256, 0, c->st>>>(c->d_tab);
   k_pix2lmn<<<blocks_for(c->npix), 256, 0, c->st>>>(c->nside, c->npix, c->d_l, c->d_m, c->d_n, c->d_nm1);
   KC();
   if (c->TP == 256) k_tile_cone<256><<<(unsigned)c->ntile, 256, 0, c->st>>>(c->d_l, c->d_m, c->d_n, c->npix, c->ntile, c->d_cone);
@@ -852,6 +855,7 @@ int enq_viss_fact(Ctx* c, int64_t tile0, int64_t nt) {
                                                               c->nfr_pad, c->d_bits_v);
   KC();
   VissFArgs a{};
+  a.tab = c->d_tab;
   a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
   a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
   a.aph = c->d_aph; a.ant8 = c->d_ant8; a.bits = c->d_bits_v; a.nfr_pad = c->nfr_pad;
@@ -984,6 +988,7 @@ int enq_viss(Ctx* c, bool count) {
     if (s.nchunk > c->nchunk_max) return c->fail(VW_ERR_RANGE, "internal: chunk buffer too small");
     if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
     VissArgs a{};
+    a.tab = c->d_tab;
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.B = c->d_B; a.Bf = c->d_Bf;
     a.npix = c->npix; a.tile0 = tile0; a.ntile = nt;
     a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls_v;
@@ -1188,6 +1193,7 @@ int enq_recon_set_one(Ctx* c, bool img, double* acc, uint32_t* wacc) {
       if (rc) return rc;
     }
     ReconArgs a{};
+    a.tab = c->d_tab;
     a.l = img ? c->d_gl : c->d_l;
     a.m = img ? c->d_gm : c->d_m;
     a.n = img ? c->d_gn : c->d_n;
@@ -1208,7 +1214,9 @@ int enq_recon_set_one(Ctx* c, bool img, double* acc, uint32_t* wacc) {
     a.P = c->P;
     a.full = c->cfg.recon_mode == VW_RECON_FULL;
     a.occ_mode = c->cfg.occ_mode;
-    a.rep_finite = !(c->cfg.recon_mode == VW_RECON_MIRROR && c->cfg.occ_mode == 2);
+    /* representatives are filtered for finite values except in the uv-mirror
+     * recon with occ_mode 2 */
+    const bool rep_finite = !(c->cfg.recon_mode == VW_RECON_MIRROR && c->cfg.occ_mode == 2);
     ReconShape rs = recon_shape(np);
     if (rs.nchunk * np > c->rpart_cap) return c->fail(VW_ERR_RANGE, "internal: recon partial buffer too small");
     a.pwaves = rs.pwaves;
@@ -1218,11 +1226,13 @@ int enq_recon_set_one(Ctx* c, bool img, double* acc, uint32_t* wacc) {
     a.wpart = img ? c->d_rwpart2 : c->d_rwpart;
     ev_begin(c, ST_RECONK, st);
     if (mixed) {
-      if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, st);
+      if (kind == VW_KIND_REP && rep_finite) launch_recon<true, VW_KIND_REP_FINITE>(a, st);
+      else if (kind == VW_KIND_REP) launch_recon<true, VW_KIND_REP>(a, st);
       else if (kind == VW_KIND_GENERIC) launch_recon<true, VW_KIND_GENERIC>(a, st);
       else launch_recon<true, VW_KIND_AVG>(a, st);
     } else {
-      if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, st);
+      if (kind == VW_KIND_REP && rep_finite) launch_recon<false, VW_KIND_REP_FINITE>(a, st);
+      else if (kind == VW_KIND_REP) launch_recon<false, VW_KIND_REP>(a, st);
       else if (kind == VW_KIND_GENERIC) launch_recon<false, VW_KIND_GENERIC>(a, st);
       else launch_recon<false, VW_KIND_AVG>(a, st);
     }
@@ -1804,6 +1814,7 @@ VW_API int vw_viss_adjoint(void* p, const vw_c128* res, double* g) {
     const int64_t nt = std::min(c->vtpp, c->ntile - tile0);
     if ((rc = enq_classes(c, c->d_cone, c->ntile, tile0, nt, false, c->d_bits_v, c->d_cls_v, c->st))) return rc;
     VissAdjArgs a{};
+    a.tab = c->d_tab;
     a.l = c->d_l; a.m = c->d_m; a.n = c->d_n; a.nm1 = c->d_nm1; a.npix = c->npix;
     a.tile0 = tile0; a.ntile = nt;
     a.rows = c->d_rows; a.rowt = c->d_rowt; a.ant8 = c->d_ant8; a.cls = (const uint32_t*)c->d_cls_v;

@@ -307,6 +307,7 @@ __global__ void k_masks(const double* __restrict__ l, const double* __restrict__
 /* Viss                                                                 */
 /* ------------------------------------------------------------------ */
 struct VissArgs {
+  const double2* tab; /* the sine table (plain forms), built once per context */
   const double *l, *m, *n, *nm1, *B;
   const float* Bf;
   int64_t npix;
@@ -321,14 +322,18 @@ struct VissArgs {
   int64_t* cpart; /* [nchunk][nh_pad] (counting instance) */
 };
 
-/* the LDS sine table, built by every workgroup of the plain kernels:
- * entry k = (cos, sin)(2 pi k / N), as vw_sincos_tab_build */
-__device__ __forceinline__ void vw_tab_fill(double2* tab) {
-  for (int i = threadIdx.x; i < VW_TAB_N; i += blockDim.x) {
-    double e[2]; /* (cos, sin) as vw_sincos_tab_build */
-    vw_sincos_turn((double)i / (double)VW_TAB_N, &e[1], &e[0]);
-    tab[i] = make_double2(e[0], e[1]);
-  }
+/* the sine table: entry k = (cos, sin)(2 pi k / N), as vw_sincos_tab_build;
+ * built once per context in device memory, copied into LDS by every
+ * workgroup of the plain kernels (four 16-byte loads a thread) */
+__global__ void k_tab_build(double2* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= VW_TAB_N) return;
+  double e[2]; /* (cos, sin) */
+  vw_sincos_turn((double)i / (double)VW_TAB_N, &e[1], &e[0]);
+  tab[i] = make_double2(e[0], e[1]);
+}
+__device__ __forceinline__ void vw_tab_fill(double2* tab, const double2* __restrict__ src) {
+  for (int i = threadIdx.x; i < VW_TAB_N; i += blockDim.x) tab[i] = src[i];
 }
 
 __device__ __forceinline__ void vw_hw_sincos_turn(float r, float* s, float* c) {
@@ -523,7 +528,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISS_WAV
   __shared__ double2 tab[VW_TAB_N];
   __shared__ double2 pre[4][4][2][64]; /* [wave][l, m, n, B][half][lane] */
   __shared__ double rant[4][RPW][10];  /* [wave][row][antenna a (x y z i c), b (...)] */
-  vw_tab_fill(tab);
+  vw_tab_fill(tab, a.tab);
   __syncthreads();
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t chunk = vb / a.bpc;
@@ -658,6 +663,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_VISS_WAV
  * wave-uniform, their class one byte of a uniform class word.  Partials
  * [chunk][pixel of the pass]; the fold applies the pixel area. */
 struct VissAdjArgs {
+  const double2* tab; /* the sine table (plain forms), built once per context */
   const double *l, *m, *n, *nm1;
   int64_t npix;
   int64_t tile0, ntile; /* tiles of this pass: [tile0, tile0 + ntile) */
@@ -676,7 +682,7 @@ template <int TP>
 __global__ __launch_bounds__(256) void k_viss_adj(VissAdjArgs a) {
   constexpr int PPL = 4;
   __shared__ double2 tab[VW_TAB_N];
-  vw_tab_fill(tab);
+  vw_tab_fill(tab, a.tab);
   __syncthreads();
   const int wave = vw_rfl((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int64_t wv = (int64_t)blockIdx.x * 4 + wave;
@@ -821,6 +827,7 @@ __device__ __forceinline__ vw_f4 vw_mfma(float a, float b, vw_f4 c) {
 }
 
 struct VissFArgs {
+  const double2* tab; /* the sine table (plain forms), built once per context */
   const double *l, *m, *n, *nm1, *B;
   const float* Bf;
   int64_t npix;
@@ -962,7 +969,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 
   constexpr int EPT = TP / 256; /* pixels staged per thread */
   __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
   __shared__ VfTileLds<TP> px;
-  if (!MIXED) vw_tab_fill(tab);
+  if (!MIXED) vw_tab_fill(tab, a.tab);
   const int64_t vb = vw_xcd_block(blockIdx.x, gridDim.x);
   const int64_t chunk = vb / a.nquad;
   if (chunk >= a.nchunk) return; /* whole workgroup */
@@ -1367,6 +1374,7 @@ __global__ void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, i
 /* reconstruction                                                       */
 /* ------------------------------------------------------------------ */
 struct ReconArgs {
+  const double2* tab; /* the sine table (plain forms), built once per context */
   const double *l, *m, *n;
   int64_t pt0, npts, TP; /* points [pt0, pt0 + npts) of this pass, pt0 = tile0 * TP */
   const uint32_t* gcls; /* [ntile_out][nq_pad]: per group (representative kind) */
@@ -1381,7 +1389,6 @@ struct ReconArgs {
   const double* ant8;
   const DevInfo* info;
   int P, full, occ_mode;
-  int rep_finite;     /* representative kind: skip groups whose value is not finite */
   int64_t nchunk;     /* group ranges per point block                     */
   int64_t pwaves;     /* point blocks (waves) per group range             */
   int64_t pstride;    /* partial row stride                               */
@@ -1392,6 +1399,8 @@ struct ReconArgs {
 #define VW_KIND_REP 0
 #define VW_KIND_GENERIC 1
 #define VW_KIND_AVG 2
+#define VW_KIND_REP_FINITE 3 /* representatives, groups with a non-finite value skipped */
+#define VW_KIND_IS_REP(k) ((k) == VW_KIND_REP || (k) == VW_KIND_REP_FINITE)
 
 /* visibility of one row for the PPL points of a lane given the class */
 template <int PPL, bool BRANCHFREE>
@@ -1492,7 +1501,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   __shared__ double2 tab[MIXED ? 1 : VW_TAB_N];
   __shared__ double gring[4][2 * 64]; /* per wave: two blocks of 16 group records */
   if (!MIXED) {
-    vw_tab_fill(tab);
+    vw_tab_fill(tab, a.tab);
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
@@ -1533,7 +1542,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   uint32_t wu = 0;
   int nblk = 0;
 
-  if (KIND == VW_KIND_REP || KIND == VW_KIND_AVG) {
+  if (VW_KIND_IS_REP(KIND) || KIND == VW_KIND_AVG) {
     /* records pass through a per-wave LDS ring in blocks of 16 (512 B: one
      * double per lane, loaded one block ahead); each group is then read
      * with a uniform LDS address.  Class words (4 groups each) are scalar
@@ -1543,7 +1552,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     const double* recd = (const double*)a.g.rec;
     double stage = (qa_ < qb_) ? recd[qa_ * 4 + lane] : 0.0;
     uint32_t ncw = 0x01010101u; /* AVG: every group counts, no class */
-    if (KIND == VW_KIND_REP && qa_ < qb_) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + qa_) >> 2]);
+    if (VW_KIND_IS_REP(KIND) && qa_ < qb_) ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + qa_) >> 2]);
     int buf = 0;
     for (int64_t q16 = qa_; q16 < qb_; q16 += 16, buf ^= 1) {
       ring[buf * 64 + lane] = stage;
@@ -1555,7 +1564,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
       const int64_t qe = q16 + 16 < qb_ ? q16 + 16 : qb_;
       for (int64_t q4 = q16; q4 < qe; q4 += 4) {
         const uint32_t cw = ncw;
-        if (KIND == VW_KIND_REP && q4 + 4 < qb_)
+        if (VW_KIND_IS_REP(KIND) && q4 + 4 < qb_)
           ncw = (uint32_t)vw_rfl((int)a.gcls[(tile * a.nq_pad + q4 + 4) >> 2]);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1563,7 +1572,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
           const int cls = (int)((cw >> (8 * j)) & 0xffu);
           if ((int)q < (int)q0 || (int)q >= (int)qb_ || cls == VW_CLS_HIDDEN) continue;
           const GRec g = blk[q - q16];
-          if (KIND == VW_KIND_REP && a.rep_finite && !(isfinite(g.vr) && isfinite(g.vi))) continue;
+          if (KIND == VW_KIND_REP_FINITE && !(isfinite(g.vr) && isfinite(g.vi))) continue;
           double vr[PPL], vi[PPL];
 #pragma unroll
           for (int k = 0; k < PPL; ++k) { vr[k] = g.vr; vi[k] = g.vi; }
