@@ -563,7 +563,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VW_RECONB_W
       xq = a.gperm[j0_ + t_];                                                            \
       xr = a.orec[j0_ + t_];                                                             \
     } else if (t_ < 34 && a.rep) {                                                       \
-      xc = *(const vw_u32x4*)(a.gcls + tile * a.nq_pad + j0_ + 16 * (t_ - 32));          \
+      /* the lane's offset is formed here each step (opaque thread index):     \
+       * held across the loop it was spilled at this kernel's register budget */ \
+      int tc_ = t_ - 32;                                                                 \
+      __asm__ volatile("" : "+v"(tc_));                                                  \
+      xc = *(const vw_u32x4*)(a.gcls + tile * a.nq_pad + j0_ + 16 * tc_);                \
     }                                                                                    \
     if (a.rep) {                                                                         \
       xf0 = a.oant[j0_ * 10 + t_];                                                       \
