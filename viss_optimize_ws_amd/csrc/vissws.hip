@@ -385,7 +385,7 @@ int make_plan(Ctx* c, const vw_config* cfg) {
 #define VW_RECON_MAX_CHUNKS 128
 #endif
 #ifndef VW_RECON_MIN_CHUNKS
-#define VW_RECON_MIN_CHUNKS 16
+#define VW_RECON_MIN_CHUNKS 24
 #endif
 
 /* Viss launch shape for nh half rows over a pass of nt tiles */
@@ -464,9 +464,9 @@ ReconShape recon_shape(int64_t npts) {
   r.pwaves = (npts + 64 * RECON_PPL - 1) / (64 * RECON_PPL);
   const int64_t target_waves = VW_RECON_TARGET_WAVES;
   r.nchunk = std::max<int64_t>(1, std::min<int64_t>(VW_RECON_MAX_CHUNKS, (target_waves + r.pwaves - 1) / r.pwaves));
-  /* large point sets get at least VW_RECON_MIN_CHUNKS group ranges while the
-   * launch stays within twice the target (Nside 256: 16 ranges, not 11,
-   * recon -3% per launch); the largest keep the target's count */
+  /* large point sets get up to VW_RECON_MIN_CHUNKS group ranges while the
+   * launch stays within twice the target (Nside 256: 21 ranges, not 11,
+   * recon -4 to -5% per launch); the largest keep the target's count */
   const int64_t lo = std::min<int64_t>(VW_RECON_MIN_CHUNKS, 2 * target_waves / r.pwaves);
   r.nchunk = std::max(r.nchunk, lo);
   return r;
