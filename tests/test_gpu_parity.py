@@ -153,6 +153,17 @@ def test_large_segment_groups(recon_mode):
     assert rel_err(Cg, Co) <= PLAIN_TOL
 
 
+def test_long_segment_frame_rows():
+    # 32 antennas x 1,040 epochs in one segment: 33,280 antenna frames per
+    # tile, a cone-bit row longer than the 32 KB the group classes stage in
+    # LDS, so they gather the endpoint bits from global memory instead
+    cfg = config(nside=8, n_ant=32, n_epochs=1040, segs=1, img_n=64, freq_hz=1e7)
+    (Cg, Wg, Ig, IWg), (Co, Wo, Io, IWo) = run_both(cfg, _sky(12 * 8 * 8, seed=11), _seg_checks(PLAIN_TOL))
+    assert np.array_equal(Wg, Wo) and np.array_equal(IWg, IWo)
+    assert rel_err(Cg, Co) <= PLAIN_TOL
+    assert rel_err(Ig, Io) <= PLAIN_TOL
+
+
 @pytest.mark.parametrize("budget,variant", [(1, dict()), (7000, dict(recon_mode=1, occ_mode=3)),
                                            (7000, dict(precision=1))])
 def test_tile_passes(budget, variant):

@@ -133,6 +133,7 @@ struct Ctx {
    * (rows on the Viss stream, groups on the grouping stream) */
   int32_t *d_okey[2] = {}, *d_oidx[2] = {}, *d_okey2[2] = {}, *d_order[2] = {};
   double2* d_orec = nullptr; /* (ugu, vgu) of the group at each work-order position */
+  int32_t *d_oia = nullptr, *d_oib = nullptr; /* its representative's antenna frames */
   double* d_oant = nullptr;  /* its representative's antenna frames [position][10]    */
   int32_t* d_rshist[2] = {};  /* radix-sort digit tables per stream (Viss, grouping)   */
   int64_t rshist_cap = 0;
@@ -402,6 +403,9 @@ struct VissShape {
 #ifndef VW_VISS_RPW_MIXED
 #define VW_VISS_RPW_MIXED 16
 #endif
+/* frame rows up to this many bytes are staged in LDS by the group classes
+ * (within the 64 KB of dynamic LDS a launch may ask for by default) */
+#define VW_GCLS_LDS_MAX 32768
 constexpr int viss_rpw(int /*TP*/, bool mixed, bool count) {
   return (mixed ? VW_VISS_RPW_MIXED : VW_VISS_RPW_PLAIN) / (count ? 2 : 1);
 }
@@ -634,7 +638,8 @@ int create_batch(Ctx* c) {
         (rc = dalloc(c, &c->d_okey2[w], (size_t)n)) || (rc = dalloc(c, &c->d_order[w], (size_t)n)))
       return rc;
   }
-  if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad)) || (rc = dalloc(c, &c->d_oant, 10 * (size_t)c->nq_pad)))
+  if ((rc = dalloc(c, &c->d_orec, (size_t)c->nq_pad)) || (rc = dalloc(c, &c->d_oant, 10 * (size_t)c->nq_pad)) ||
+      (rc = dalloc(c, &c->d_oia, (size_t)c->nq_pad)) || (rc = dalloc(c, &c->d_oib, (size_t)c->nq_pad)))
     return rc;
   {
     const int64_t nmax = std::max<int64_t>(c->nq_pad, 2 * c->max_nh);
@@ -832,10 +837,17 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
   if (nt * c->nfr_pad > c->bits_cap) return c->fail(VW_ERR_RANGE, "internal: cone bit buffer too small");
   k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad, bits);
   KC();
-  if (groups)
-    k_group_cls<<<blocks_for(nt * c->nq_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->gs.gia, c->gs.gib, c->d_info,
-                                                            c->nq_pad, work_order ? c->d_order[1] : nullptr, cls);
-  else
+  if (groups) {
+    const int nqb = (int)((c->nq_pad + VW_GCLS_BLOCK - 1) / VW_GCLS_BLOCK);
+    const unsigned nb = (unsigned)(nt * nqb);
+    const int32_t* ia = work_order ? c->d_oia : c->gs.gia;
+    const int32_t* ib = work_order ? c->d_oib : c->gs.gib;
+    if (c->nfr_pad <= VW_GCLS_LDS_MAX)
+      k_group_cls<true><<<nb, 256, (size_t)c->nfr_pad, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad,
+                                                              (uint32_t*)cls);
+    else
+      k_group_cls<false><<<nb, 256, 0, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad, (uint32_t*)cls);
+  } else
     k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, cls);
   KC();
   return VW_OK;
@@ -1101,8 +1113,9 @@ int enq_group(Ctx* c) {
   if (c->batch) {
     int rc;
     if ((rc = enq_order(c, 1))) return rc;
-    k_group_order_recs<<<blocks_for(c->nq_pad), 256, 0, c->stB>>>(c->d_order[1], c->gs.rec, c->gs.ant, c->nq_pad,
-                                                                  c->d_orec, c->d_oant);
+    k_group_order_recs<<<blocks_for(c->nq_pad), 256, 0, c->stB>>>(c->d_order[1], c->gs.rec, c->gs.ant, c->gs.gia,
+                                                                  c->gs.gib, c->nq_pad, c->d_orec, c->d_oant,
+                                                                  c->d_oia, c->d_oib);
     KC();
     k_group_pack<<<blocks_for(c->nq_pad * c->nsky_pad), 256, 0, c->stB>>>(
         c->d_goff, c->d_idx_sorted, c->d_vh, c->nh, c->P, full ? 1 : 0, c->cfg.blockage, c->cfg.occ_mode, c->d_info,

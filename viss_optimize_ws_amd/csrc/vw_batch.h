@@ -135,15 +135,19 @@ __global__ void k_row_keys(const RowRec* __restrict__ rows, const double* __rest
   keys[i] = i < nh ? vw_pair_key(ant8, rows[i].ia, rows[i].ib) : 4096;
   idx[i] = (int32_t)i;
 }
-/* cell centres and representative antenna frames in the recon's work order */
+/* cell centres, representative antenna frames and their frame indices in
+ * the recon's work order */
 __global__ void k_group_order_recs(const int32_t* __restrict__ order, const GRec* __restrict__ rec,
-                                   const double* __restrict__ gant, int64_t n, double2* __restrict__ orec,
-                                   double* __restrict__ oant) {
+                                   const double* __restrict__ gant, const int32_t* __restrict__ gia,
+                                   const int32_t* __restrict__ gib, int64_t n, double2* __restrict__ orec,
+                                   double* __restrict__ oant, int32_t* __restrict__ oia, int32_t* __restrict__ oib) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int64_t q = order[j];
   const GRec g = rec[q];
   orec[j] = make_double2(g.ugu, g.vgu);
+  oia[j] = gia[q];
+  oib[j] = gib[q];
 #pragma unroll
   for (int f = 0; f < 10; ++f) oant[j * 10 + f] = gant[q * 10 + f];
 }

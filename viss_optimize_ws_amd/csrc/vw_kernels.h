@@ -1355,19 +1355,51 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
   d[5] = qb[0]; d[6] = qb[1]; d[7] = qb[2]; d[8] = qb[3]; d[9] = qb[7];
 }
 
-/* class of (output tile, group) from the representative's antennas; with an
- * order (batched recon) entry j of a tile's row holds group order[j] */
-__global__ void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, int64_t nt,
-                            const int32_t* __restrict__ gia, const int32_t* __restrict__ gib,
-                            const DevInfo* __restrict__ info, int64_t nq_pad, const int32_t* __restrict__ order,
-                            uint8_t* __restrict__ gcls) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nt * nq_pad) return;
-  int64_t tr = k / nq_pad, q = k - tr * nq_pad;
-  if (order) q = order[q];
-  if (q >= info->nuniq) { gcls[k] = VW_CLS_HIDDEN; return; }
-  const uint8_t* b = bits + tr * nfr_pad;
-  gcls[k] = (uint8_t)vw_pair_class(b[gia[q]], b[gib[q]]);
+/* class of (output tile, group) from the representative's antennas: entry j
+ * of a tile's row pairs frames ia[j] and ib[j] (the groups' own arrays, or
+ * for the batched recon the same arrays in its work order, which puts the
+ * groups past nuniq last, so in both entries j >= nuniq are hidden).  A
+ * block covers VW_GCLS_BLOCK entries of one tile's row, four a thread
+ * written as one class word (byte e = entry 4 i + e).  Each thread issues
+ * its index loads first, then the block stages that tile's frame row of
+ * cone bits in LDS (STAGE; nfr_pad bytes of dynamic LDS; longer rows are
+ * gathered from global memory), so the loads overlap.  Blocks that start
+ * past the 64-entry block holding entry nuniq - 1 write nothing: the recon
+ * kernels read the classes of entries below nuniq only, in words of 4 (one
+ * sky) or blocks of 32 (batched) aligned to their size. */
+#define VW_GCLS_BLOCK 1024
+template <bool STAGE>
+__global__ __launch_bounds__(256) void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, int nqb,
+                                                   const int32_t* __restrict__ ia, const int32_t* __restrict__ ib,
+                                                   const DevInfo* __restrict__ info, int64_t nq_pad,
+                                                   uint32_t* __restrict__ gcls) {
+  extern __shared__ uint4 sbits[];
+  static_assert(VW_GCLS_BLOCK == 4 * 256, "four entries a thread");
+  const uint32_t tr = blockIdx.x / (uint32_t)nqb, qc = blockIdx.x - tr * (uint32_t)nqb;
+  const int64_t j00 = (int64_t)qc * VW_GCLS_BLOCK;
+  const int64_t nu = info->nuniq;
+  if (j00 >= (nu + 63) / 64 * 64) return;
+  const uint8_t* row = bits + (int64_t)tr * nfr_pad;
+  const int64_t j0 = j00 + 4 * threadIdx.x;
+  const bool live = j0 < nq_pad; /* nq_pad is a multiple of 64: whole words only */
+  const int64_t jl = live ? j0 : 0;
+  /* indices past nuniq are not frames: gather frame 0 for them instead */
+  const int4 a4 = *(const int4*)(ia + jl), b4 = *(const int4*)(ib + jl);
+  const int av[4] = {a4.x, a4.y, a4.z, a4.w}, bv[4] = {b4.x, b4.y, b4.z, b4.w};
+  if (STAGE) {
+    /* nfr_pad is a multiple of 64 and the row starts 64-byte aligned */
+    for (int64_t i = threadIdx.x; i < nfr_pad / 16; i += 256) sbits[i] = ((const uint4*)row)[i];
+    __syncthreads();
+  }
+  const uint8_t* b = STAGE ? (const uint8_t*)sbits : row;
+  uint32_t w = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const bool grp = jl + e < nu;
+    const uint32_t cls = (uint32_t)vw_pair_class(b[grp ? av[e] : 0], b[grp ? bv[e] : 0]);
+    w |= (grp ? cls : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
+  }
+  if (live) gcls[((int64_t)tr * nq_pad + j0) >> 2] = w;
 }
 
 /* ------------------------------------------------------------------ */
