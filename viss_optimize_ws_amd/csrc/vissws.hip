@@ -829,6 +829,17 @@ int enq_gen(Ctx* c, int t0, int tlen) {
   return VW_OK;
 }
 
+/* row classes of nt tiles into cls ([nt][nh_pad] bytes) from their cone bits */
+void launch_classify(Ctx* c, const uint8_t* bits, int64_t nt, uint8_t* cls, hipStream_t st) {
+  const int nhb = (int)((c->nh_pad + VW_GCLS_BLOCK - 1) / VW_GCLS_BLOCK);
+  const unsigned nb = (unsigned)(nt * nhb);
+  if (c->nfr_pad <= VW_GCLS_LDS_MAX)
+    k_classify<true><<<nb, 256, (size_t)c->nfr_pad, st>>>(bits, c->nfr_pad, nhb, c->d_rows, c->nh, c->nh_pad,
+                                                           (uint32_t*)cls);
+  else
+    k_classify<false><<<nb, 256, 0, st>>>(bits, c->nfr_pad, nhb, c->d_rows, c->nh, c->nh_pad, (uint32_t*)cls);
+}
+
 /* class table of a tile pass into the class scratch: cone bits per
  * (tile, antenna frame), then per (tile, half row) or (tile, group) */
 int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups,
@@ -848,7 +859,7 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
     else
       k_group_cls<false><<<nb, 256, 0, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad, (uint32_t*)cls);
   } else
-    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, cls);
+    launch_classify(c, bits, nt, cls, st);
   KC();
   return VW_OK;
 }
@@ -2047,7 +2058,7 @@ int full_classes(Ctx* c, bool img, std::vector<uint8_t>& t, uint8_t** dev_out) {
   if (e == hipSuccess) {
     k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, c->st>>>(img ? c->d_gcone : c->d_cone, nt, 0, nt, c->d_ant8,
                                                                (int64_t)c->tlen * c->A, c->nfr_pad, bits);
-    k_classify<<<blocks_for(nt * c->nh_pad), 256, 0, c->st>>>(bits, c->nfr_pad, nt, c->d_rows, c->nh, c->nh_pad, d);
+    launch_classify(c, bits, nt, d, c->st);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(t.data(), d, t.size(), hipMemcpyDeviceToHost, c->st);

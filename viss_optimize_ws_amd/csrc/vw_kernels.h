@@ -267,17 +267,55 @@ __global__ void k_ant_bits(const double* __restrict__ cone, int64_t ntile, int64
                                   cone[3 * ntile + t], cone[4 * ntile + t], q[7]);
 }
 
-/* pair class per (tile, half row) for the nt tiles of a pass -> bytes
- * [nt][nh_pad] (rows past nh: hidden), from the frame bits */
-__global__ void k_classify(const uint8_t* __restrict__ bits, int64_t nfr_pad, int64_t nt,
-                           const RowRec* __restrict__ rows, int64_t nh, int64_t nh_pad, uint8_t* __restrict__ cls) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nt * nh_pad) return;
-  int64_t tr = k / nh_pad, h = k - tr * nh_pad;
-  if (h >= nh) { cls[k] = VW_CLS_HIDDEN; return; }
-  const RowRec r = rows[h];
-  const uint8_t* b = bits + tr * nfr_pad;
-  cls[k] = (uint8_t)vw_pair_class(b[r.ia], b[r.ib]);
+/* pair class per (tile, half row) for the tiles of a pass -> bytes
+ * [tiles][nh_pad] (rows past nh: hidden), from the frame bits.  A block
+ * covers VW_GCLS_BLOCK rows of one tile, four a thread written as one class
+ * word.  The rows' frame indices pass through LDS (loaded with consecutive
+ * lanes on consecutive records, read back four rows a thread), beside the
+ * tile's row of cone bits staged in LDS (STAGE; nfr_pad bytes of dynamic
+ * LDS) or, for longer rows, gathered from global memory, as k_group_cls. */
+#define VW_GCLS_BLOCK 1024
+template <bool STAGE>
+__global__ __launch_bounds__(256) void k_classify(const uint8_t* __restrict__ bits, int64_t nfr_pad, int nhb,
+                                                  const RowRec* __restrict__ rows, int64_t nh, int64_t nh_pad,
+                                                  uint32_t* __restrict__ cls) {
+  extern __shared__ uint4 sbits[];
+  __shared__ int4 sab[VW_GCLS_BLOCK / 2]; /* (ia, ib) of rows 2 k and 2 k + 1 */
+  static_assert(VW_GCLS_BLOCK == 4 * 256, "four entries a thread");
+  const uint32_t tr = blockIdx.x / (uint32_t)nhb, hc = blockIdx.x - tr * (uint32_t)nhb;
+  const int64_t hb = (int64_t)hc * VW_GCLS_BLOCK;
+  const uint8_t* row = bits + (int64_t)tr * nfr_pad;
+  /* rows past nh are not frames: frame 0 stands in (the row array holds at
+   * least one record) */
+  int2* sab2 = (int2*)sab;
+  int2 ab[4];
+  int keep[4]; /* all ones for a row, zero past nh (a mask, not a select, so
+                * the four loads stay unconditional and in flight together) */
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t h = hb + threadIdx.x + 256 * k;
+    keep[k] = -(int)(h < nh);
+    ab[k] = *(const int2*)&rows[h < nh ? h : 0].ia;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sab2[threadIdx.x + 256 * k] = make_int2(ab[k].x & keep[k], ab[k].y & keep[k]);
+  if (STAGE) {
+    /* nfr_pad is a multiple of 64 and the row starts 64-byte aligned */
+    for (int64_t i = threadIdx.x; i < nfr_pad / 16; i += 256) sbits[i] = ((const uint4*)row)[i];
+  }
+  __syncthreads();
+  const int64_t h0 = hb + 4 * threadIdx.x;
+  if (h0 >= nh_pad) return; /* nh_pad is a multiple of 64: whole words only */
+  const int4 p0 = sab[2 * threadIdx.x], p1 = sab[2 * threadIdx.x + 1];
+  const int av[4] = {p0.x, p0.z, p1.x, p1.z}, bv[4] = {p0.y, p0.w, p1.y, p1.w};
+  const uint8_t* b = STAGE ? (const uint8_t*)sbits : row;
+  uint32_t w = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t c = (uint32_t)vw_pair_class(b[av[e]], b[bv[e]]);
+    w |= (h0 + e < nh ? c : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
+  }
+  cls[((int64_t)tr * nh_pad + h0) >> 2] = w;
 }
 
 /* effective mask of rows [row0, row0+nrows) x pixels (introspection) */
@@ -1367,7 +1405,6 @@ __global__ void k_group_data(const int64_t* __restrict__ gkeys, const int32_t* _
  * past the 64-entry block holding entry nuniq - 1 write nothing: the recon
  * kernels read the classes of entries below nuniq only, in words of 4 (one
  * sky) or blocks of 32 (batched) aligned to their size. */
-#define VW_GCLS_BLOCK 1024
 template <bool STAGE>
 __global__ __launch_bounds__(256) void k_group_cls(const uint8_t* __restrict__ bits, int64_t nfr_pad, int nqb,
                                                    const int32_t* __restrict__ ia, const int32_t* __restrict__ ib,
