@@ -715,11 +715,20 @@ __global__ void k_reconb_fold(const double* __restrict__ part, const uint32_t* _
   if (k >= nsky * npts) return;
   const int64_t s = k / npts, p = k - s * npts;
   double v = part[s * pstride + p];
-  for (int64_t c = 1; c < nchunk; ++c) v = v + part[(c * nsky_pad + s) * pstride + p];
+  /* VW_FOLD_BATCH chunks' loads in flight at a time; the sum keeps chunk order */
+  int64_t c = 1;
+  for (; c + VW_FOLD_BATCH <= nchunk; c += VW_FOLD_BATCH) {
+    double x[VW_FOLD_BATCH];
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) x[e] = part[((c + e) * nsky_pad + s) * pstride + p];
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) v = v + x[e];
+  }
+  for (; c < nchunk; ++c) v = v + part[(c * nsky_pad + s) * pstride + p];
   acc[s * npts_all + pt0 + p] += v;
   if (weights && s == 0) {
     uint32_t w = wpart[p];
-    for (int64_t c = 1; c < nchunk; ++c) w += wpart[c * pstride + p];
+    for (int64_t c2 = 1; c2 < nchunk; ++c2) w += wpart[c2 * pstride + p];
     wacc[pt0 + p] += w;
   }
 }

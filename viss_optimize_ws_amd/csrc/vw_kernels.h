@@ -37,6 +37,9 @@ struct RowRec {
   int32_t ia, ib; /* antenna-frame indices g*A + a of the two endpoints */
 };
 
+/* chunk partials a fold kernel loads before adding them (in chunk order) */
+#define VW_FOLD_BATCH 8
+
 struct DevInfo {
   vw_segparams sp;
   int64_t nuniq;
@@ -808,7 +811,23 @@ __global__ void k_viss_reduce(const double2* __restrict__ part, int64_t nchunk, 
   if (h >= nh) return;
   double re = 0.0, im = 0.0;
   int64_t c = 0;
-  for (int64_t k = 0; k < nchunk; ++k) {
+  /* VW_FOLD_BATCH chunks' loads in flight at a time; the sums keep chunk order */
+  int64_t k = 0;
+  for (; k + VW_FOLD_BATCH <= nchunk; k += VW_FOLD_BATCH) {
+    double2 v[VW_FOLD_BATCH];
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) v[e] = part[(k + e) * nh_pad + h];
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) {
+      re += v[e].x;
+      im += v[e].y;
+    }
+    if (counts) {
+#pragma unroll
+      for (int e = 0; e < VW_FOLD_BATCH; ++e) c += cpart[(k + e) * nh_pad + h];
+    }
+  }
+  for (; k < nchunk; ++k) {
     double2 v = part[k * nh_pad + h];
     re += v.x;
     im += v.y;
@@ -1742,7 +1761,23 @@ __global__ void k_recon_reduce(const double* __restrict__ part, const uint32_t* 
   if (p >= npts) return;
   double s = part[p];
   uint32_t w = wpart[p];
-  for (int64_t k = 1; k < nchunk; ++k) {
+  /* VW_FOLD_BATCH chunks' loads in flight at a time; the sum keeps chunk order */
+  int64_t k = 1;
+  for (; k + VW_FOLD_BATCH <= nchunk; k += VW_FOLD_BATCH) {
+    double x[VW_FOLD_BATCH];
+    uint32_t y[VW_FOLD_BATCH];
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) {
+      x[e] = part[(k + e) * pstride + p];
+      y[e] = wpart[(k + e) * pstride + p];
+    }
+#pragma unroll
+    for (int e = 0; e < VW_FOLD_BATCH; ++e) {
+      s = s + x[e];
+      w += y[e];
+    }
+  }
+  for (; k < nchunk; ++k) {
     s = s + part[k * pstride + p];
     w += wpart[k * pstride + p];
   }
