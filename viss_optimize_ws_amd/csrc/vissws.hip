@@ -845,6 +845,20 @@ void launch_classify(Ctx* c, const uint8_t* bits, int64_t nt, uint8_t* cls, hipS
 int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, int64_t nt, bool groups,
                 uint8_t* bits, uint8_t* cls, hipStream_t st, bool work_order = false) {
   const int64_t nfr = (int64_t)c->tlen * c->A;
+  if (c->nfr_pad <= VW_GCLS_LDS_MAX) {
+    /* one block per tile makes the tile's cone bits in LDS: no bit table */
+    const size_t lds = (size_t)c->nfr_pad;
+    if (groups)
+      k_group_cls_tile<<<(unsigned)nt, 256, lds, st>>>(cone, ntile_all, tile0, c->d_ant8, nfr,
+                                                       work_order ? c->d_oia : c->gs.gia,
+                                                       work_order ? c->d_oib : c->gs.gib, c->d_info, c->nq_pad,
+                                                       (uint32_t*)cls);
+    else
+      k_classify_tile<<<(unsigned)nt, 256, lds, st>>>(cone, ntile_all, tile0, c->d_ant8, nfr, c->d_rows, c->nh,
+                                                      c->nh_pad, (uint32_t*)cls);
+    KC();
+    return VW_OK;
+  }
   if (nt * c->nfr_pad > c->bits_cap) return c->fail(VW_ERR_RANGE, "internal: cone bit buffer too small");
   k_ant_bits<<<blocks_for(nt * c->nfr_pad), 256, 0, st>>>(cone, ntile_all, tile0, nt, c->d_ant8, nfr, c->nfr_pad, bits);
   KC();

@@ -270,6 +270,17 @@ __global__ void k_ant_bits(const double* __restrict__ cone, int64_t ntile, int64
                                   cone[3 * ntile + t], cone[4 * ntile + t], q[7]);
 }
 
+/* the cone bits of tile t's nfr frames (as k_ant_bits) into LDS */
+__device__ __forceinline__ void vw_tile_bits_lds(uint8_t* sb, const double* __restrict__ cone, int64_t ntile,
+                                                 int64_t t, const double* __restrict__ ant8, int64_t nfr) {
+  const double cx = cone[t], cy = cone[ntile + t], cz = cone[2 * ntile + t];
+  const double cA = cone[3 * ntile + t], sA = cone[4 * ntile + t];
+  for (int64_t f = threadIdx.x; f < nfr; f += blockDim.x) {
+    const double* q = ant8 + f * 8;
+    sb[f] = (uint8_t)vw_cone_bits(q[4], q[5], q[6], cx, cy, cz, cA, sA, q[7]);
+  }
+}
+
 /* pair class per (tile, half row) for the tiles of a pass -> bytes
  * [tiles][nh_pad] (rows past nh: hidden), from the frame bits.  A block
  * covers VW_GCLS_BLOCK rows of one tile, four a thread written as one class
@@ -319,6 +330,48 @@ __global__ __launch_bounds__(256) void k_classify(const uint8_t* __restrict__ bi
     w |= (h0 + e < nh ? c : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
   }
   cls[((int64_t)tr * nh_pad + h0) >> 2] = w;
+}
+
+/* k_classify with the cone bits made in place: one block per tile of the
+ * pass computes the tile's frame bits into LDS (nfr bytes of dynamic LDS),
+ * then walks the tile's rows VW_GCLS_BLOCK at a time as k_classify does, so
+ * no bit table passes through global memory and each cone test runs once */
+__global__ __launch_bounds__(256) void k_classify_tile(const double* __restrict__ cone, int64_t ntile,
+                                                       int64_t tile0, const double* __restrict__ ant8, int64_t nfr,
+                                                       const RowRec* __restrict__ rows, int64_t nh, int64_t nh_pad,
+                                                       uint32_t* __restrict__ cls) {
+  extern __shared__ uint4 sbits[];
+  __shared__ int4 sab[VW_GCLS_BLOCK / 2]; /* (ia, ib) of rows 2 k and 2 k + 1 */
+  const int64_t tr = blockIdx.x;
+  uint8_t* sb = (uint8_t*)sbits;
+  vw_tile_bits_lds(sb, cone, ntile, tile0 + tr, ant8, nfr);
+  int2* sab2 = (int2*)sab;
+  for (int64_t hb = 0; hb < nh_pad; hb += VW_GCLS_BLOCK) {
+    int2 ab[4];
+    int keep[4]; /* a mask, not a select: the four loads stay in flight together */
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t h = hb + threadIdx.x + 256 * k;
+      keep[k] = -(int)(h < nh);
+      ab[k] = *(const int2*)&rows[h < nh ? h : 0].ia;
+    }
+    __syncthreads(); /* the bits are in; the last pass's reads of sab are done */
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sab2[threadIdx.x + 256 * k] = make_int2(ab[k].x & keep[k], ab[k].y & keep[k]);
+    __syncthreads();
+    const int64_t h0 = hb + 4 * threadIdx.x;
+    if (h0 < nh_pad) {
+      const int4 p0 = sab[2 * threadIdx.x], p1 = sab[2 * threadIdx.x + 1];
+      const int av[4] = {p0.x, p0.z, p1.x, p1.z}, bv[4] = {p0.y, p0.w, p1.y, p1.w};
+      uint32_t w = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t c = (uint32_t)vw_pair_class(sb[av[e]], sb[bv[e]]);
+        w |= (h0 + e < nh ? c : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
+      }
+      cls[(tr * nh_pad + h0) >> 2] = w;
+    }
+  }
 }
 
 /* effective mask of rows [row0, row0+nrows) x pixels (introspection) */
@@ -1456,6 +1509,39 @@ __global__ __launch_bounds__(256) void k_group_cls(const uint8_t* __restrict__ b
     w |= (grp ? cls : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
   }
   if (live) gcls[((int64_t)tr * nq_pad + j0) >> 2] = w;
+}
+
+/* k_group_cls with the cone bits made in place: one block per tile of the
+ * pass computes the tile's frame bits into LDS (nfr bytes of dynamic LDS),
+ * then writes the class words of entries below nuniq rounded up to 64 (the
+ * entries the recon kernels read), VW_GCLS_BLOCK at a time */
+__global__ __launch_bounds__(256) void k_group_cls_tile(const double* __restrict__ cone, int64_t ntile,
+                                                        int64_t tile0, const double* __restrict__ ant8, int64_t nfr,
+                                                        const int32_t* __restrict__ ia,
+                                                        const int32_t* __restrict__ ib,
+                                                        const DevInfo* __restrict__ info, int64_t nq_pad,
+                                                        uint32_t* __restrict__ gcls) {
+  extern __shared__ uint4 sbits[];
+  const int64_t tr = blockIdx.x;
+  const int64_t nu = info->nuniq;
+  const int64_t qend = (nu + 63) / 64 * 64 < nq_pad ? (nu + 63) / 64 * 64 : nq_pad;
+  if (qend == 0) return;
+  uint8_t* sb = (uint8_t*)sbits;
+  vw_tile_bits_lds(sb, cone, ntile, tile0 + tr, ant8, nfr);
+  __syncthreads();
+  for (int64_t j0 = 4 * threadIdx.x; j0 < qend; j0 += VW_GCLS_BLOCK) {
+    /* indices past nuniq are not frames: frame 0 stands in for them */
+    const int4 a4 = *(const int4*)(ia + j0), b4 = *(const int4*)(ib + j0);
+    const int av[4] = {a4.x, a4.y, a4.z, a4.w}, bv[4] = {b4.x, b4.y, b4.z, b4.w};
+    uint32_t w = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool grp = j0 + e < nu;
+      const uint32_t cls = (uint32_t)vw_pair_class(sb[grp ? av[e] : 0], sb[grp ? bv[e] : 0]);
+      w |= (grp ? cls : (uint32_t)VW_CLS_HIDDEN) << (8 * e);
+    }
+    gcls[(tr * nq_pad + j0) >> 2] = w;
+  }
 }
 
 /* ------------------------------------------------------------------ */
