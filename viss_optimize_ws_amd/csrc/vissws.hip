@@ -867,11 +867,8 @@ int enq_classes(Ctx* c, const double* cone, int64_t ntile_all, int64_t tile0, in
     const unsigned nb = (unsigned)(nt * nqb);
     const int32_t* ia = work_order ? c->d_oia : c->gs.gia;
     const int32_t* ib = work_order ? c->d_oib : c->gs.gib;
-    if (c->nfr_pad <= VW_GCLS_LDS_MAX)
-      k_group_cls<true><<<nb, 256, (size_t)c->nfr_pad, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad,
-                                                              (uint32_t*)cls);
-    else
-      k_group_cls<false><<<nb, 256, 0, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad, (uint32_t*)cls);
+    /* frame rows here are longer than the LDS staging limit */
+    k_group_cls<false><<<nb, 256, 0, st>>>(bits, c->nfr_pad, nqb, ia, ib, c->d_info, c->nq_pad, (uint32_t*)cls);
   } else
     launch_classify(c, bits, nt, cls, st);
   KC();
